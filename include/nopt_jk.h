@@ -1,0 +1,104 @@
+/* RI-J/K Fock build on one MI355X (gfx950): the C-ABI boundary.
+ *
+ * This replaces NOPT's RI layer (include/RI.h:6-23, src/RI.cpp) for the
+ * Fock build. NOPT keeps module globals (RI_AA, AUX_Vm05, RI_CA, RI_CA2,
+ * src/RI.cpp:16-32); here that state lives in a context that owns the B
+ * tensor resident in HBM.
+ *
+ * Conventions
+ *  - Every matrix is row-major FP64 on the host side unless stated.
+ *  - Densities D are normalised to 1 as in NOPT: D = sum_i C_i C_i^T over the
+ *    occupied rows of the orbital matrix (gen_HF_DM, src/SCF.cpp:63-73); the
+ *    closed-shell Fock matrix is F = H + 2J - K (molecule::calc_F_AO,
+ *    src/molecule.cpp:4407-4438).
+ *  - Orbital factors are passed as NOPT passes MO: rows are orbitals,
+ *    C[i*nbf + m], and K is built from the first nocc rows
+ *    (DM_to_F_transform_RI, src/RI.cpp:146-171).
+ *  - Return value: 0 (NOPT_OK) or a NOPT_E* code; nopt_last_error() gives the
+ *    message. NOPT itself exits or throws on these conditions
+ *    (src/RI.cpp:36-39, std::bad_alloc); here they are reported.
+ *  - Calls are host-synchronous unless they take a stream; one context must
+ *    not be used from two threads at once.
+ */
+#ifndef NOPT_JK_H
+#define NOPT_JK_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NOPT_OK 0
+#define NOPT_EINVAL 1 /* bad argument or shape */
+#define NOPT_ENOMEM 2 /* device or host allocation failed */
+#define NOPT_ECHOL 3  /* Coulomb metric not positive definite */
+#define NOPT_EHIP 4   /* HIP runtime error */
+#define NOPT_EIO 5    /* file could not be read */
+#define NOPT_ESTATE 6 /* call out of order (e.g. a build before set_B) */
+
+/* Arithmetic of the half-transform and exchange contraction. J, the metric
+ * and the Fock assembly are FP64 in every mode. */
+#define NOPT_PREC_FP64 0   /* FP64 MFMA */
+#define NOPT_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
+#define NOPT_PREC_BF16X3 2 /* bf16 hi/lo split, three MFMA products, FP32 accumulation */
+
+typedef struct nopt_jk_ctx nopt_jk_ctx;
+
+/* Creates a context on HIP device `device` for nbf orbital functions and naux
+ * auxiliary functions. Nothing is allocated on the device until set_B.
+ * Replaces the allocation part of gen_RI_AA (src/RI.cpp:34-79). */
+int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out);
+void nopt_jk_destroy(nopt_jk_ctx* ctx);
+
+/* Forms the fitted tensor B = L^-1 (P|mn) on the device in FP64, where
+ * V = L L^T is the Cholesky factor of the Coulomb metric, and keeps it
+ * resident. eri3 is packed [P][m(m+1)/2+n] (m >= n), metric is naux x naux.
+ * NOPT instead keeps raw (mn|P) with V^-1/2 from an eigendecomposition and
+ * applies it on every build (src/RI.cpp:57-59, 152-156, src/matr.cpp:372-401);
+ * any W with W^T W = V^-1 gives the same J and K. Fails with NOPT_ECHOL when
+ * the metric is not positive definite (NOPT drops such eigenvalues with a
+ * warning, src/matr.cpp:384-386). */
+int nopt_jk_set_B(nopt_jk_ctx* ctx, const double* eri3_packed, const double* metric);
+
+/* Loads an already fitted B (packed [Q][m(m+1)/2+n], FP64) without the metric
+ * step. */
+int nopt_jk_set_B_fitted(nopt_jk_ctx* ctx, const double* B_packed);
+
+/* Copies the fitted B back to the host (packed layout, FP64). */
+int nopt_jk_get_B(nopt_jk_ctx* ctx, double* B_packed);
+
+/* Device bytes held by the context. */
+long nopt_jk_device_bytes(const nopt_jk_ctx* ctx);
+
+/* One RI-J/K build, host buffers: J_mn = sum_Q B_Qmn sum_rs B_Qrs D_rs and
+ * K_mn = sum_Q sum_{i<nocc} (sum_r B_Qmr C_ir)(sum_s B_Qns C_is).
+ * J and K (nbf x nbf) are overwritten. Either output may be NULL to skip it;
+ * D may be NULL when J is NULL and C may be NULL when K is NULL.
+ * Replaces DM_to_F_transform_RI (src/RI.cpp:115-176). */
+int nopt_jk_build(nopt_jk_ctx* ctx, const double* D, const double* C, long nocc, double* J,
+                  double* K, int prec);
+
+/* The same build on device buffers, enqueued on `stream` (a hipStream_t, NULL
+ * for the default stream); returns after enqueueing. */
+int nopt_jk_build_device(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc,
+                         double* dJ, double* dK, int prec, void* stream);
+
+/* Fock assembly and energy on the device from host buffers:
+ * F = H + cj*J - ck*K and E = sum D.*F + sum D.*H (E_1el_calc,
+ * src/doCI_matr.cpp:38-47, as used at src/SCF.cpp:134; the caller adds
+ * V_nuc). F may be NULL; E may be NULL. */
+int nopt_jk_fock_energy(nopt_jk_ctx* ctx, const double* H, const double* D, const double* J,
+                        const double* K, double cj, double ck, double* F, double* E);
+
+/* Closed-shell energy and its orbital gradient from the occupied factor:
+ * D = C^T C (first nocc rows), F = H + 2J - K, E = sum D.*(H+F), and
+ * G = dE/dC = 4 C F (nocc x nbf, rows are orbitals). This is the adjoint of
+ * E_RHF(C); its occupied-virtual projection is NOPT's orbital gradient
+ * g_ia = 4 F_ai (src/converger_2_1.cpp:54-62). G may be NULL. */
+int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, long nocc,
+                            int prec, double* E, double* G);
+
+const char* nopt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
