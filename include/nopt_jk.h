@@ -96,6 +96,24 @@ int nopt_jk_fock_energy(nopt_jk_ctx* ctx, const double* H, const double* D, cons
 int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, long nocc,
                             int prec, double* E, double* G);
 
+/* Per-phase timing with HIP events on the build's stream (no counterpart in
+ * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When
+ * enabled, each build records events around its five phases: copy-in,
+ * half-transform (with the fused gamma partials), Coulomb, exchange and
+ * finalize. read_timing returns the summed milliseconds per phase (ms[5])
+ * and the number of builds recorded since the last read, then resets. */
+int nopt_jk_set_timing(nopt_jk_ctx* ctx, int enable);
+int nopt_jk_read_timing(nopt_jk_ctx* ctx, double* ms, long* builds);
+/* The context's own stream (hipStream_t). */
+void* nopt_jk_stream(nopt_jk_ctx* ctx);
+
+/* Device-memory plumbing for callers without a HIP runtime binding.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+void* nopt_dev_malloc(int device, size_t bytes);
+void nopt_dev_free(void* ptr);
+int nopt_dev_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int nopt_dev_synchronize(int device);
+
 const char* nopt_last_error(void);
 
 #ifdef __cplusplus

@@ -55,6 +55,13 @@ GPU_SYMBOLS = [
                                            ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     ("nopt_jk_energy_gradient", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_int,
                                                _c_double_p, _c_double_p]),
+    ("nopt_jk_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("nopt_jk_read_timing", ctypes.c_int, [_vp, _c_double_p, _c_long_p]),
+    ("nopt_jk_stream", _vp, [_vp]),
+    ("nopt_dev_malloc", _vp, [ctypes.c_int, ctypes.c_size_t]),
+    ("nopt_dev_free", None, [_vp]),
+    ("nopt_dev_memcpy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
+    ("nopt_dev_synchronize", ctypes.c_int, [ctypes.c_int]),
 ]
 
 

@@ -1,0 +1,1291 @@
+// RI-J/K Fock build for gfx950: kernels, device context and the C entry
+// points of include/nopt_jk.h.
+//
+// Replaces DM_to_F_transform_RI (src/RI.cpp:115-176). NOPT's sequence is
+//   gamma = D.RI_AA; c = V^-1/2 V^-1/2 gamma; J = RI_AA.c;
+//   X = MO.RI_AA; X~ = X.V^-1/2; transpose; K = Y.Y^T (full GEMM).
+// Here the metric is folded into B once (B = L^-1 (P|mn), linalg.hip), so a
+// build is
+//   half-transform   X[Q][m][i] = sum_n B[Q][m][n] C[i][n]   (MFMA GEMM)
+//                    with gamma_Q = sum_mn B[Q][m][n] D[m][n] fused into the
+//                    same pass over B (per-tile partial sums)
+//   Coulomb          J[m][n] = sum_Q gamma_Q B[Q][m][n]      (HBM-bound)
+//   exchange         K[m][n] = sum_Q sum_i X[Q][m][i] X[Q][n][i]  (MFMA
+//                    SYRK over lower-triangle tiles, split over Q)
+//   finalize         deterministic reduction of the Q-split partials in FP64
+//                    and symmetric write of J and K.
+//
+// HBM layout (row-major): B64[Q][m][ld64] FP64, full square with zero padding
+// columns (ld64 a multiple of 16); the mixed-precision kernels round or split
+// it on the fly; X[Q][m][ldx] in the build's
+// precision; J/K partials FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../../include/nopt_jk.h"
+#include "../host/errors.hpp"
+#include "common.hpp"
+#include "linalg.hpp"
+
+using nopt::set_error;
+
+namespace {
+
+constexpr int BM = 128;     // half-transform rows per tile
+constexpr int KT = 128;     // exchange tile edge
+constexpr int NSPLIT_J_MIN = 8;  // minimum Q-split of the Coulomb pass
+constexpr int JCHUNK_MAX = 4096; // gamma values cached in LDS per split
+
+// ------------------------------------------------------------------ copies
+__global__ void k_unpack_B(const double* __restrict__ Bp, long npair, int nbf, long nrow_total,
+                           double* __restrict__ B64, long ld64) {
+  // one thread per (Q, m, n) with n < ld64
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nrow_total * ld64) return;
+  const long row = e / ld64;  // Q*nbf + m
+  const int n = (int)(e % ld64);
+  const long Q = row / nbf;
+  const int m = (int)(row % nbf);
+  double v = 0.0;
+  if (n < nbf) {
+    const int a = m > n ? m : n, b = m > n ? n : m;
+    v = Bp[Q * npair + (long)a * (a + 1) / 2 + b];
+  }
+  B64[row * ld64 + n] = v;
+}
+
+__global__ void k_pack_B(const double* __restrict__ B64, long ld64, int nbf, long npair, long naux,
+                         double* __restrict__ Bp) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= naux * npair) return;
+  const long Q = e / npair, p = e % npair;
+  long a = (long)((std::sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while ((a + 1) * (a + 2) / 2 <= p) ++a;
+  while (a * (a + 1) / 2 > p) --a;
+  const long b = p - a * (a + 1) / 2;
+  Bp[e] = B64[(Q * nbf + a) * ld64 + b];
+}
+
+// dst[r][c] (ldd) = src[r][c] (lds) for r < rows, c < cols; zero elsewhere
+// in [0, rows_pad) x [0, ldd)
+template <typename T>
+__global__ void k_pad_copy(const double* __restrict__ src, long lds, int rows, int cols,
+                           T* __restrict__ dst, long ldd, int rows_pad) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)rows_pad * ldd) return;
+  const long r = e / ldd, c = e % ldd;
+  double v = (r < rows && c < cols) ? src[r * lds + c] : 0.0;
+  dst[e] = (T)v;
+}
+
+// --------------------------------------------------- half-transform f64/f32
+// X[Q][m][col] = sum_n B[Q][m][n] Cp[col][n] for one (m-tile, Q, column group)
+// and gpart[Q][mtile] = sum over the tile of B[Q][m][n] D[m][n].
+template <typename T>
+struct Vec8 {
+  T v[8];
+};
+
+template <typename T>
+__device__ __forceinline__ void load8(const T* p, T* out) {
+  if constexpr (sizeof(T) == 8) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      double2 x = q[i];
+      out[2 * i] = x.x;
+      out[2 * i + 1] = x.y;
+    }
+  } else {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      float4 x = q[i];
+      out[4 * i] = x.x;
+      out[4 * i + 1] = x.y;
+      out[4 * i + 2] = x.z;
+      out[4 * i + 3] = x.w;
+    }
+  }
+}
+
+// B is always read in FP64; the FP32 variant rounds it while staging into LDS,
+// so the fused gamma partials are FP64 in both variants.
+template <typename T, int NT>
+__global__ __launch_bounds__(256) void k_half(const double* __restrict__ B, long ld, int nbf,
+                                              const T* __restrict__ Cp, const double* __restrict__ Dp,
+                                              T* __restrict__ X, long ldx, double* __restrict__ gpart,
+                                              int mtiles) {
+  constexpr int BN = 16 * NT, BK = 16;
+  using acc_t = typename std::conditional<sizeof(T) == 8, d4_t, f4_t>::type;
+  __shared__ T As[BK][BM + 2];
+  __shared__ T Cs[BK][BN + 2];
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int mtile = blockIdx.x;
+  const long Q = blockIdx.y;
+  const int cg = blockIdx.z;
+  const int m0 = mtile * BM;
+  const int ar = t >> 1, ak = (t & 1) * 8;
+  const bool arow_ok = (m0 + ar) < nbf;
+  const double* Arow = B + ((long)Q * nbf + m0 + ar) * ld + ak;
+  const double* Drow = Dp + (long)(m0 + ar) * ld + ak;
+  const bool c_ok = t < 2 * BN;
+  const T* Crow = Cp + ((long)cg * BN + (t >> 1)) * ld + ak;
+  acc_t acc[2][NT];
+  d4_t acc64[2][NT];  // used by the FP32 variant only
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+      acc64[i][j] = d4_t{0, 0, 0, 0};
+    }
+  double rd = 0.0;
+  double a_reg[8];
+  T c_reg[8];
+  double d_reg[8];
+  auto gload = [&](int k0) {
+    if (arow_ok) {
+      load8<double>(Arow + k0, a_reg);
+      const double2* dq = reinterpret_cast<const double2*>(Drow + k0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double2 x = dq[i];
+        d_reg[2 * i] = x.x;
+        d_reg[2 * i + 1] = x.y;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        a_reg[i] = 0;
+        d_reg[i] = 0;
+      }
+    }
+    if (c_ok) load8<T>(Crow + k0, c_reg);
+  };
+  gload(0);
+  for (int k0 = 0; k0 < ld; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      As[ak + i][ar] = (T)a_reg[i];
+      rd += a_reg[i] * d_reg[i];
+    }
+    if (c_ok) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Cs[ak + i][t >> 1] = c_reg[i];
+    }
+    __syncthreads();
+    if (k0 + BK < ld) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      T a[2], b[NT];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[ks + lane / 16][w * 32 + i * 16 + lane % 16];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) b[j] = Cs[ks + lane / 16][j * 16 + lane % 16];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          if constexpr (sizeof(T) == 8)
+            acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
+          else
+            acc[i][j] = mfma_f32(a[i], b[j], acc[i][j]);
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+      // FP32 chains of BK terms, accumulated across chains in FP64
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = acc_t{0, 0, 0, 0};
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + w * 32 + i * 16 + acc_row<T>(lane, v);
+      if (m < nbf) {
+        T* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          if constexpr (sizeof(T) == 4)
+            xr[j * 16] = (T)acc64[i][j][v];
+          else
+            xr[j * 16] = acc[i][j][v];
+        }
+      }
+    }
+  if (cg == 0) {
+    double s = block_sum256(rd, red);
+    if (t == 0) gpart[Q * mtiles + mtile] = s;
+  }
+}
+
+// ------------------------------------------------------ half-transform bf16x3
+// B (FP64) and C (FP32) are split into bf16 hi/lo while they are staged in
+// LDS; each 16x16x32 step issues lo*hi, hi*lo and hi*hi products into one FP32
+// accumulator. X is written in FP32. ld is a multiple of 16; the 32-wide
+// k-steps mask the tail.
+template <int NT>
+__global__ __launch_bounds__(256) void k_half_bf16x3(const double* __restrict__ B, long ld, int nbf,
+                                                     const float* __restrict__ Cp,
+                                                     const double* __restrict__ Dp,
+                                                     float* __restrict__ X, long ldx,
+                                                     double* __restrict__ gpart, int mtiles) {
+  constexpr int BN = 16 * NT, BK = 32, PK = BK + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[BM][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Al[BM][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Ch[BN][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Cl[BN][PK];
+  __shared__ double red[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int mtile = blockIdx.x;
+  const long Q = blockIdx.y;
+  const int cg = blockIdx.z;
+  const int m0 = mtile * BM;
+  const int ar = t >> 1, ak = (t & 1) * 16;
+  const bool arow_ok = (m0 + ar) < nbf;
+  const double* Arow = B + ((long)Q * nbf + m0 + ar) * ld + ak;
+  const double* Drow = Dp + (long)(m0 + ar) * ld + ak;
+  const bool c_ok = t < 2 * BN;
+  const float* Crow = Cp + ((long)cg * BN + (t >> 1)) * ld + ak;
+  f4_t acc[2][NT];
+  d4_t acc64[2][NT];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      acc[i][j] = f4_t{0, 0, 0, 0};
+      acc64[i][j] = d4_t{0, 0, 0, 0};
+    }
+  double rd = 0.0;
+  double a_reg[16], d_reg[16];
+  float c_reg[16];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool kin = k0 + ak + 8 * h + 8 <= ld;
+      if (arow_ok && kin) {
+        load8<double>(Arow + k0 + 8 * h, a_reg + 8 * h);
+        load8<double>(Drow + k0 + 8 * h, d_reg + 8 * h);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          a_reg[8 * h + i] = 0.0;
+          d_reg[8 * h + i] = 0.0;
+        }
+      }
+      if (c_ok) {
+        if (kin)
+          load8<float>(Crow + k0 + 8 * h, c_reg + 8 * h);
+        else
+#pragma unroll
+          for (int i = 0; i < 8; ++i) c_reg[8 * h + i] = 0.f;
+      }
+    }
+  };
+  gload(0);
+  for (int k0 = 0; k0 < ld; k0 += BK) {
+    __syncthreads();
+    {
+      uint16_t h[16], l[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        split_bf16((float)a_reg[i], h[i], l[i]);
+        rd += a_reg[i] * d_reg[i];
+      }
+      uint4* ph = reinterpret_cast<uint4*>(&Ah[ar][ak]);
+      uint4* pl = reinterpret_cast<uint4*>(&Al[ar][ak]);
+      ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
+      ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
+      pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
+      pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
+    }
+    if (c_ok) {
+      uint16_t h[16], l[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) split_bf16(c_reg[i], h[i], l[i]);
+      uint4* ph = reinterpret_cast<uint4*>(&Ch[t >> 1][ak]);
+      uint4* pl = reinterpret_cast<uint4*>(&Cl[t >> 1][ak]);
+      ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
+      ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
+      pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
+      pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
+    }
+    __syncthreads();
+    if (k0 + BK < ld) gload(k0 + BK);
+    uint4 ah[2], al[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int r = w * 32 + i * 16 + lane % 16;
+      ah[i] = *reinterpret_cast<const uint4*>(&Ah[r][8 * (lane / 16)]);
+      al[i] = *reinterpret_cast<const uint4*>(&Al[r][8 * (lane / 16)]);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int c = j * 16 + lane % 16;
+      const uint4 bh = *reinterpret_cast<const uint4*>(&Ch[c][8 * (lane / 16)]);
+      const uint4 bl = *reinterpret_cast<const uint4*>(&Cl[c][8 * (lane / 16)]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+        acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+        acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+      }
+    }
+    // FP32 chain of one 32-wide k-step, accumulated in FP64
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
+        acc[i][j] = f4_t{0, 0, 0, 0};
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + w * 32 + i * 16 + 4 * (lane / 16) + v;
+      if (m < nbf) {
+        float* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) xr[j * 16] = (float)acc64[i][j][v];
+      }
+    }
+  if (cg == 0) {
+    double s = block_sum256(rd, red);
+    if (t == 0) gpart[Q * mtiles + mtile] = s;
+  }
+}
+
+// ------------------------------------------------------------------ gamma
+__global__ void k_gamma(const double* __restrict__ gpart, int mtiles, int naux,
+                        double* __restrict__ gamma) {
+  const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Q >= naux) return;
+  double s = 0.0;
+  for (int i = 0; i < mtiles; ++i) s += gpart[(long)Q * mtiles + i];
+  gamma[Q] = s;
+}
+
+// ---------------------------------------------------------------- Coulomb
+// Jpart[s][m][n] = sum_{Q in split s} gamma_Q B[Q][m][n] over the lower
+// triangle. A block owns the row pair (p, nbf-1-p), nbf+1 elements.
+template <typename T>
+__global__ __launch_bounds__(256) void k_jpart(const T* __restrict__ B, long ld, int nbf, int naux,
+                                               const double* __restrict__ gamma,
+                                               double* __restrict__ Jpart) {
+  constexpr int EPT = 8;  // elements per thread: nbf + 1 <= 2048
+  __shared__ double g[JCHUNK_MAX];
+  const int p = blockIdx.x, s = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) g[q - q0] = gamma[q];
+  __syncthreads();
+  const int m1 = p, m2 = nbf - 1 - p;
+  const int n1 = m1 + 1, cnt = (m2 != m1) ? n1 + m2 + 1 : n1;
+  int row[EPT], col[EPT];
+  double acc[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    const int e = threadIdx.x + j * 256;
+    acc[j] = 0.0;
+    if (e < n1) {
+      row[j] = m1;
+      col[j] = e;
+    } else if (e < cnt) {
+      row[j] = m2;
+      col[j] = e - n1;
+    } else {
+      row[j] = -1;
+      col[j] = 0;
+    }
+  }
+  for (int q = q0; q < q1; ++q) {
+    const double gq = g[q - q0];
+    const T* Bq = B + (long)q * nbf * ld;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j)
+      if (row[j] >= 0) acc[j] += gq * (double)Bq[(long)row[j] * ld + col[j]];
+  }
+  double* Js = Jpart + (long)s * nbf * nbf;
+#pragma unroll
+  for (int j = 0; j < EPT; ++j)
+    if (row[j] >= 0) Js[(long)row[j] * nbf + col[j]] = acc[j];
+}
+
+// --------------------------------------------------------------- exchange
+__device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
+  bi = (int)((std::sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  bj = t - bi * (bi + 1) / 2;
+}
+
+// Kpart[s][tile][r][c] = sum_{Q in split s} sum_i X[Q][bi*KT+r][i] X[Q][bj*KT+c][i]
+template <typename T>
+__global__ __launch_bounds__(256) void k_syrk(const T* __restrict__ X, long ldx, int nbf, int naux,
+                                              int kpad, double* __restrict__ Kpart) {
+  constexpr int BK = 16;
+  using acc_t = typename std::conditional<sizeof(T) == 8, d4_t, f4_t>::type;
+  __shared__ T As[BK][KT + 2];
+  __shared__ T Bs[BK][KT + 2];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int s = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
+  const int r = t >> 1, kk = (t & 1) * 8;
+  const bool ra = bi * KT + r < nbf, rb = bj * KT + r < nbf;
+  acc_t acc[4][4];
+  d4_t acc64[4][4];  // used by the FP32 variant only
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = acc_t{0, 0, 0, 0};
+      acc64[i][j] = d4_t{0, 0, 0, 0};
+    }
+  T a_reg[8], b_reg[8];
+  auto gload = [&](int q, int i0) {
+    const T* xa = X + ((long)q * nbf + bi * KT + r) * ldx + i0 + kk;
+    const T* xb = X + ((long)q * nbf + bj * KT + r) * ldx + i0 + kk;
+    if (ra)
+      load8<T>(xa, a_reg);
+    else
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a_reg[i] = 0;
+    if (rb)
+      load8<T>(xb, b_reg);
+    else
+#pragma unroll
+      for (int i = 0; i < 8; ++i) b_reg[i] = 0;
+  };
+  if (q0 < q1) gload(q0, 0);
+  for (int q = q0; q < q1; ++q) {
+    for (int i0 = 0; i0 < kpad; i0 += BK) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        As[kk + i][r] = a_reg[i];
+        Bs[kk + i][r] = b_reg[i];
+      }
+      __syncthreads();
+      int nq = q, ni = i0 + BK;
+      if (ni >= kpad) {
+        ni = 0;
+        ++nq;
+      }
+      if (nq < q1) gload(nq, ni);
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        T a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = As[ks + lane / 16][wr * 64 + i * 16 + lane % 16];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = Bs[ks + lane / 16][wc * 64 + j * 16 + lane % 16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            if constexpr (sizeof(T) == 8)
+              acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
+            else
+              acc[i][j] = mfma_f32(a[i], b[j], acc[i][j]);
+          }
+      }
+      if constexpr (sizeof(T) == 4) {
+        // FP32 chains of BK terms, accumulated in FP64
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
+            acc[i][j] = acc_t{0, 0, 0, 0};
+          }
+      }
+    }
+  }
+  double* Kt = Kpart + ((long)s * gridDim.x + blockIdx.x) * KT * KT;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = wr * 64 + i * 16 + acc_row<T>(lane, v);
+        const int cc = wc * 64 + j * 16 + lane % 16;
+        if constexpr (sizeof(T) == 4)
+          Kt[rr * KT + cc] = acc64[i][j][v];
+        else
+          Kt[rr * KT + cc] = acc[i][j][v];
+      }
+}
+
+// bf16x3 exchange: X (FP32) is split into hi/lo while staged in LDS.
+__global__ __launch_bounds__(256) void k_syrk_bf16x3(const float* __restrict__ X, long ldx, int nbf,
+                                                     int naux, int kpad,
+                                                     double* __restrict__ Kpart) {
+  constexpr int BK = 32, PK = BK + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[KT][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Al[KT][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[KT][PK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bl[KT][PK];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int s = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
+  const int r = t >> 1, kk = (t & 1) * 16;
+  const bool ra = bi * KT + r < nbf, rb = bj * KT + r < nbf;
+  f4_t acc[4][4];
+  d4_t acc64[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = f4_t{0, 0, 0, 0};
+      acc64[i][j] = d4_t{0, 0, 0, 0};
+    }
+  float a_reg[16], b_reg[16];
+  auto gload = [&](int q, int i0) {
+    const float* xa = X + ((long)q * nbf + bi * KT + r) * ldx + i0 + kk;
+    const float* xb = X + ((long)q * nbf + bj * KT + r) * ldx + i0 + kk;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int base = i0 + kk + 8 * h;
+      const bool inr = base + 8 <= ldx;  // ldx is a multiple of 16
+      if (ra && inr)
+        load8<float>(xa + 8 * h, a_reg + 8 * h);
+      else
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a_reg[8 * h + i] = 0.f;
+      if (rb && inr)
+        load8<float>(xb + 8 * h, b_reg + 8 * h);
+      else
+#pragma unroll
+        for (int i = 0; i < 8; ++i) b_reg[8 * h + i] = 0.f;
+    }
+  };
+  auto stage = [&](uint16_t (*H)[PK], uint16_t (*L)[PK], const float* v) {
+    uint16_t h[16], l[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) split_bf16(v[i], h[i], l[i]);
+    uint4* ph = reinterpret_cast<uint4*>(&H[r][kk]);
+    uint4* pl = reinterpret_cast<uint4*>(&L[r][kk]);
+    ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
+    ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
+    pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
+    pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
+  };
+  if (q0 < q1) gload(q0, 0);
+  for (int q = q0; q < q1; ++q) {
+    for (int i0 = 0; i0 < kpad; i0 += BK) {
+      __syncthreads();
+      stage(Ah, Al, a_reg);
+      stage(Bh, Bl, b_reg);
+      __syncthreads();
+      int nq = q, ni = i0 + BK;
+      if (ni >= kpad) {
+        ni = 0;
+        ++nq;
+      }
+      if (nq < q1) gload(nq, ni);
+      uint4 ah[4], al[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rr = wr * 64 + i * 16 + lane % 16;
+        ah[i] = *reinterpret_cast<const uint4*>(&Ah[rr][8 * (lane / 16)]);
+        al[i] = *reinterpret_cast<const uint4*>(&Al[rr][8 * (lane / 16)]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int cc = wc * 64 + j * 16 + lane % 16;
+        const uint4 bh = *reinterpret_cast<const uint4*>(&Bh[cc][8 * (lane / 16)]);
+        const uint4 bl = *reinterpret_cast<const uint4*>(&Bl[cc][8 * (lane / 16)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
+          acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
+          acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
+        }
+      }
+      // FP32 chain of one 32-wide k-step, accumulated in FP64
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
+          acc[i][j] = f4_t{0, 0, 0, 0};
+        }
+    }
+  }
+  double* Kt = Kpart + ((long)s * gridDim.x + blockIdx.x) * KT * KT;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = wr * 64 + i * 16 + 4 * (lane / 16) + v;
+        const int cc = wc * 64 + j * 16 + lane % 16;
+        Kt[rr * KT + cc] = acc64[i][j][v];
+      }
+}
+
+// --------------------------------------------------------------- finalize
+// J = sum_s Jpart[s], K = sum_s Kpart[s] over the lower triangle, written to
+// both triangles of J, K (ld nbf). Either output may be null.
+__global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const double* __restrict__ Kpart,
+                           int nsk, int ntiles, int nbf, double* __restrict__ J,
+                           double* __restrict__ K) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)nbf * nbf) return;
+  const int m = (int)(e / nbf), n = (int)(e % nbf);
+  if (n > m) return;
+  if (J) {
+    double s = 0.0;
+    for (int k = 0; k < nsj; ++k) s += Jpart[((long)k * nbf + m) * nbf + n];
+    J[(long)m * nbf + n] = s;
+    J[(long)n * nbf + m] = s;
+  }
+  if (K) {
+    const int bi = m / KT, bj = n / KT;
+    const int tile = bi * (bi + 1) / 2 + bj;
+    const int r = m % KT, c = n % KT;
+    double s = 0.0;
+    for (int k = 0; k < nsk; ++k) s += Kpart[(((long)k * ntiles + tile) * KT + r) * KT + c];
+    K[(long)m * nbf + n] = s;
+    K[(long)n * nbf + m] = s;
+  }
+}
+
+// ----------------------------------------------------------- Fock, energy
+__global__ void k_fock_energy(const double* __restrict__ H, const double* __restrict__ D,
+                              const double* __restrict__ J, const double* __restrict__ K, double cj,
+                              double ck, long nn, double* __restrict__ F,
+                              double* __restrict__ epart) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < nn; e += (long)gridDim.x * blockDim.x) {
+    const double h = H[e];
+    double f = h;
+    if (J) f += cj * J[e];
+    if (K) f -= ck * K[e];
+    if (F) F[e] = f;
+    acc += D[e] * (f + h);
+  }
+  double s = block_sum256(acc, red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = s;
+}
+
+__global__ void k_sum(const double* __restrict__ v, int n, double* __restrict__ out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += v[i];
+  double s = block_sum256(acc, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// D[m][n] = sum_i C[i][m] C[i][n]
+__global__ void k_density(const double* __restrict__ C, int nocc, int nbf, double* __restrict__ D) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)nbf * nbf) return;
+  const int m = (int)(e / nbf), n = (int)(e % nbf);
+  double s = 0.0;
+  for (int i = 0; i < nocc; ++i) s += C[(long)i * nbf + m] * C[(long)i * nbf + n];
+  D[e] = s;
+}
+
+}  // namespace
+
+// ================================================================ context
+struct nopt_jk_ctx {
+  int device = 0;
+  long nbf = 0, naux = 0, npair = 0;
+  long ld64 = 0;
+  int mtiles = 0, ktn = 0, ntiles = 0;
+  hipStream_t stream = nullptr;
+  double* B64 = nullptr;
+  // per-build work buffers (grown on demand)
+  long cap_ldx = 0;
+  void* X = nullptr;  // naux*nbf*ldx elements of the largest type
+  size_t X_bytes = 0;
+  double *gpart = nullptr, *gamma = nullptr, *Jpart = nullptr, *Kpart = nullptr;
+  int nsk = 0, nsj = 0;
+  double *Dp = nullptr;  // nbf x ld64 padded density
+  void* Cp = nullptr;    // padded orbital factor
+  size_t Cp_bytes = 0;
+  double *dJ = nullptr, *dK = nullptr, *dD = nullptr, *dC = nullptr, *dH = nullptr,
+         *dF = nullptr, *dG = nullptr, *epart = nullptr, *dE = nullptr;
+  long dC_cap = 0;
+  int* info = nullptr;
+  // optional per-phase timing: events for up to kTimedBuilds builds
+  bool timing = false;
+  int timed = 0;
+  std::vector<hipEvent_t> tev;
+};
+
+namespace {
+constexpr int kPhases = 5;  // copy-in, half-transform, Coulomb, exchange, finalize
+constexpr int kTimedBuilds = 512;
+}  // namespace
+
+namespace {
+
+const char* hip_msg(hipError_t e) { return hipGetErrorString(e); }
+
+#define HIPCHK(call)                                                                       \
+  do {                                                                                     \
+    hipError_t _e = (call);                                                                \
+    if (_e != hipSuccess)                                                                  \
+      return set_error(_e == hipErrorOutOfMemory ? NOPT_ENOMEM : NOPT_EHIP,                \
+                       std::string(#call) + ": " + hip_msg(_e));                          \
+  } while (0)
+
+int dev_alloc(void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return set_error(NOPT_ENOMEM, std::string("device allocation of ") + std::to_string(bytes) +
+                                      " bytes for " + what + " failed: " + hip_msg(e));
+  }
+  return NOPT_OK;
+}
+
+template <typename P>
+void dev_free(P*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// Grows a device buffer to at least `bytes`.
+int ensure(void** p, size_t* cap, size_t bytes, const char* what) {
+  if (*cap >= bytes && *p) return NOPT_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  int rc = dev_alloc(p, bytes, what);
+  if (rc == NOPT_OK) *cap = bytes;
+  return rc;
+}
+
+int nt_for(long nocc, int* ngroups) {
+  const long t = (nocc + 15) / 16;
+  int nt = t <= 1 ? 1 : t <= 2 ? 2 : t <= 4 ? 4 : t <= 6 ? 6 : 8;
+  *ngroups = (int)((t + nt - 1) / nt);
+  return nt;
+}
+
+template <typename T>
+hipError_t launch_half(int nt, dim3 grid, hipStream_t s, const double* B, long ld, int nbf, const T* Cp,
+                       const double* Dp, T* X, long ldx, double* gpart, int mtiles) {
+#define NOPT_HALF(NTV) \
+  case NTV:            \
+    k_half<T, NTV><<<grid, 256, 0, s>>>(B, ld, nbf, Cp, Dp, X, ldx, gpart, mtiles); \
+    break;
+  switch (nt) {
+    NOPT_HALF(1)
+    NOPT_HALF(2)
+    NOPT_HALF(4)
+    NOPT_HALF(6)
+    NOPT_HALF(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NOPT_HALF
+  return hipGetLastError();
+}
+
+hipError_t launch_half_bf16x3(int nt, dim3 grid, hipStream_t s, const double* B, long ld, int nbf,
+                              const float* Cp, const double* Dp, float* X, long ldx, double* gpart,
+                              int mtiles) {
+#define NOPT_HALF(NTV) \
+  case NTV:            \
+    k_half_bf16x3<NTV><<<grid, 256, 0, s>>>(B, ld, nbf, Cp, Dp, X, ldx, gpart, mtiles); \
+    break;
+  switch (nt) {
+    NOPT_HALF(1)
+    NOPT_HALF(2)
+    NOPT_HALF(4)
+    NOPT_HALF(6)
+    NOPT_HALF(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef NOPT_HALF
+  return hipGetLastError();
+}
+
+int check_ctx(nopt_jk_ctx* c, bool need_B) {
+  if (!c) return set_error(NOPT_EINVAL, "null context");
+  if (need_B && !c->B64) return set_error(NOPT_ESTATE, "B tensor not set (call nopt_jk_set_B first)");
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("hipSetDevice: ") + hip_msg(e));
+  return NOPT_OK;
+}
+
+// Unpacks a packed fitted B on the device into B64.
+int finish_B(nopt_jk_ctx* c, const double* dBp) {
+  if (!c->B64) {
+    int rc = dev_alloc((void**)&c->B64, sizeof(double) * c->naux * c->nbf * c->ld64, "B (FP64)");
+    if (rc) return rc;
+  }
+  const long rows = c->naux * c->nbf;
+  const long total = rows * c->ld64;
+  k_unpack_B<<<(unsigned)((total + 255) / 256), 256, 0, c->stream>>>(dBp, c->npair, (int)c->nbf, rows,
+                                                                     c->B64, c->ld64);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NOPT_OK;
+}
+
+// Enqueues one build on device buffers. dD: nbf x nbf, dC: nocc x nbf.
+int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, double* dJ,
+                  double* dK, int prec, hipStream_t s) {
+  const int nbf = (int)c->nbf, naux = (int)c->naux;
+  const bool wantK = dK != nullptr && nocc > 0;
+  const bool wantJ = dJ != nullptr;
+  if (prec < 0 || prec > 2) return set_error(NOPT_EINVAL, "unknown precision mode");
+  if (wantK && !dC) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
+  if (wantJ && !dD) return set_error(NOPT_EINVAL, "J requested without a density");
+  if (!wantJ && !wantK) return NOPT_OK;
+  int ngroups = 1;
+  const int nt = nt_for(wantK ? nocc : 1, &ngroups);
+  const long ldx = (long)nt * 16 * ngroups;  // padded orbital count
+  const bool f64 = prec == NOPT_PREC_FP64;
+  const long ldb = c->ld64;
+  const size_t esz = f64 ? sizeof(double) : sizeof(float);
+  int rc;
+  size_t cap = c->X_bytes;
+  rc = ensure(&c->X, &cap, esz * c->naux * c->nbf * ldx, "half-transformed tensor X");
+  if (rc) return rc;
+  c->X_bytes = cap;
+  size_t cpb = c->Cp_bytes;
+  rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
+  if (rc) return rc;
+  c->Cp_bytes = cpb;
+  const bool tm = c->timing && c->timed < kTimedBuilds;
+  auto mark = [&](int ph) {
+    if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
+  };
+  mark(0);
+  // padded copies of D (FP64, ld = ldb) and C (compute type)
+  {
+    const long n = (long)nbf * ldb;
+    if (dD)
+      k_pad_copy<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dD, nbf, nbf, nbf, c->Dp, ldb, nbf);
+    else
+      k_pad_copy<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c->Dp, ldb, 0, 0, c->Dp, ldb, nbf);
+    const long nc = ldx * ldb;
+    const int rows = wantK ? (int)nocc : 0;
+    if (f64)
+      k_pad_copy<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(dC ? dC : c->Dp, nbf, rows, nbf,
+                                                                      (double*)c->Cp, ldb, (int)ldx);
+    else
+      k_pad_copy<float><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(dC ? dC : c->Dp, nbf, rows, nbf,
+                                                                     (float*)c->Cp, ldb, (int)ldx);
+    HIPCHK(hipGetLastError());
+  }
+  // half-transform (+ gamma partials)
+  mark(1);
+  dim3 gh(c->mtiles, naux, wantK ? ngroups : 1);
+  hipError_t e;
+  if (prec == NOPT_PREC_FP64)
+    e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X,
+                            ldx, c->gpart, c->mtiles);
+  else if (prec == NOPT_PREC_FP32)
+    e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
+                           c->gpart, c->mtiles);
+  else
+    e = launch_half_bf16x3(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
+                           c->gpart, c->mtiles);
+  HIPCHK(e);
+  mark(2);
+  if (wantJ) {
+    k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, c->mtiles, naux, c->gamma);
+    dim3 gj((nbf + 1) / 2, c->nsj);
+    k_jpart<double><<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, c->Jpart);
+    HIPCHK(hipGetLastError());
+  }
+  mark(3);
+  if (wantK) {
+    const int kpad = (int)round_up(nocc, prec == NOPT_PREC_BF16X3 ? 32 : 16);
+    dim3 gk(c->ntiles, c->nsk);
+    if (prec == NOPT_PREC_FP64)
+      k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X, ldx, nbf, naux, kpad, c->Kpart);
+    else if (prec == NOPT_PREC_FP32)
+      k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
+    else
+      k_syrk_bf16x3<<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
+    HIPCHK(hipGetLastError());
+  }
+  mark(4);
+  const long nn = (long)nbf * nbf;
+  k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(c->Jpart, c->nsj, c->Kpart, c->nsk,
+                                                          c->ntiles, nbf, wantJ ? dJ : nullptr,
+                                                          wantK ? dK : nullptr);
+  HIPCHK(hipGetLastError());
+  mark(5);
+  if (tm) ++c->timed;
+  return NOPT_OK;
+}
+
+int upload(nopt_jk_ctx* c, double** dst, long* cap, const double* src, long count) {
+  if (*cap < count || !*dst) {
+    dev_free(*dst);
+    int rc = dev_alloc((void**)dst, sizeof(double) * std::max(count, 1L), "host staging copy");
+    if (rc) return rc;
+    *cap = count;
+  }
+  HIPCHK(hipMemcpyAsync(*dst, src, sizeof(double) * count, hipMemcpyHostToDevice, c->stream));
+  return NOPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
+  nopt::clear_error();
+  if (!out) return set_error(NOPT_EINVAL, "nopt_jk_create: null output pointer");
+  *out = nullptr;
+  if (nbf <= 0 || naux <= 0) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf and naux must be positive");
+  if (nbf > 2046) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf above 2046 is not supported");
+  if (naux > 65535) return set_error(NOPT_EINVAL, "nopt_jk_create: naux above 65535 is not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_error(NOPT_EHIP, "nopt_jk_create: no HIP device available");
+  if (device < 0 || device >= ndev) return set_error(NOPT_EINVAL, "nopt_jk_create: bad device index");
+  auto* c = new (std::nothrow) nopt_jk_ctx;
+  if (!c) return set_error(NOPT_ENOMEM, "nopt_jk_create: host allocation failed");
+  c->device = device;
+  c->nbf = nbf;
+  c->naux = naux;
+  c->npair = nbf * (nbf + 1) / 2;
+  c->ld64 = round_up(nbf, 16);
+  c->mtiles = (int)((nbf + BM - 1) / BM);
+  c->ktn = (int)((nbf + KT - 1) / KT);
+  c->ntiles = c->ktn * (c->ktn + 1) / 2;
+  c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
+  c->nsj = std::max(NSPLIT_J_MIN, (int)((naux + JCHUNK_MAX - 1) / JCHUNK_MAX));
+  int rc = NOPT_OK;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return set_error(NOPT_EHIP, std::string("nopt_jk_create: ") + hip_msg(e));
+  }
+  const long nn = nbf * nbf;
+  if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles, "gamma partials");
+  if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * naux, "gamma");
+  if (!rc) rc = dev_alloc((void**)&c->Jpart, sizeof(double) * c->nsj * nn, "J partials");
+  if (!rc) rc = dev_alloc((void**)&c->Kpart, sizeof(double) * (long)c->nsk * c->ntiles * KT * KT, "K partials");
+  if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * nbf * c->ld64, "padded density");
+  if (!rc) rc = dev_alloc((void**)&c->dJ, sizeof(double) * nn, "J");
+  if (!rc) rc = dev_alloc((void**)&c->dK, sizeof(double) * nn, "K");
+  if (!rc) rc = dev_alloc((void**)&c->dD, sizeof(double) * nn, "D");
+  if (!rc) rc = dev_alloc((void**)&c->dH, sizeof(double) * nn, "H");
+  if (!rc) rc = dev_alloc((void**)&c->dF, sizeof(double) * nn, "F");
+  if (!rc) rc = dev_alloc((void**)&c->epart, sizeof(double) * 1024, "energy partials");
+  if (!rc) rc = dev_alloc((void**)&c->dE, sizeof(double), "energy");
+  if (!rc) rc = dev_alloc((void**)&c->info, sizeof(int), "info");
+  if (rc) {
+    nopt_jk_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return NOPT_OK;
+}
+
+void nopt_jk_destroy(nopt_jk_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->B64);
+  dev_free(c->X);
+  dev_free(c->gpart);
+  dev_free(c->gamma);
+  dev_free(c->Jpart);
+  dev_free(c->Kpart);
+  dev_free(c->Dp);
+  dev_free(c->Cp);
+  dev_free(c->dJ);
+  dev_free(c->dK);
+  dev_free(c->dD);
+  dev_free(c->dC);
+  dev_free(c->dH);
+  dev_free(c->dF);
+  dev_free(c->dG);
+  dev_free(c->epart);
+  dev_free(c->dE);
+  dev_free(c->info);
+  for (auto& ev : c->tev) (void)hipEventDestroy(ev);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int nopt_jk_set_timing(nopt_jk_ctx* c, int enable) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (enable && c->tev.empty()) {
+    c->tev.resize((size_t)kTimedBuilds * (kPhases + 1));
+    for (auto& ev : c->tev) HIPCHK(hipEventCreate(&ev));
+  }
+  c->timing = enable != 0;
+  c->timed = 0;
+  return NOPT_OK;
+}
+
+int nopt_jk_read_timing(nopt_jk_ctx* c, double* ms, long* builds) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!ms) return set_error(NOPT_EINVAL, "nopt_jk_read_timing: null output");
+  for (int p = 0; p < kPhases; ++p) ms[p] = 0.0;
+  if (!c->tev.empty())
+    for (int b = 0; b < c->timed; ++b) {
+      const hipEvent_t* e = &c->tev[(size_t)b * (kPhases + 1)];
+      HIPCHK(hipEventSynchronize(e[kPhases]));
+      for (int p = 0; p < kPhases; ++p) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e[p], e[p + 1]));
+        ms[p] += t;
+      }
+    }
+  if (builds) *builds = c->timed;
+  c->timed = 0;
+  return NOPT_OK;
+}
+
+void* nopt_jk_stream(nopt_jk_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+void* nopt_dev_malloc(int device, size_t bytes) {
+  void* p = nullptr;
+  if (hipSetDevice(device) != hipSuccess) return nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(NOPT_ENOMEM, "nopt_dev_malloc failed");
+    return nullptr;
+  }
+  return p;
+}
+
+void nopt_dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// kind: 0 host->device, 1 device->host, 2 device->device
+int nopt_dev_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  nopt::clear_error();
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                   : hipMemcpyDeviceToDevice;
+  HIPCHK(hipMemcpy(dst, src, bytes, k));
+  return NOPT_OK;
+}
+
+int nopt_dev_synchronize(int device) {
+  nopt::clear_error();
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipDeviceSynchronize());
+  return NOPT_OK;
+}
+
+int nopt_jk_set_B(nopt_jk_ctx* c, const double* eri3, const double* metric) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!eri3 || !metric) return set_error(NOPT_EINVAL, "nopt_jk_set_B: null input");
+  const long na = c->naux;
+  double *dV = nullptr, *dA = nullptr;
+  rc = dev_alloc((void**)&dV, sizeof(double) * na * na, "metric");
+  if (!rc) rc = dev_alloc((void**)&dA, sizeof(double) * na * c->npair, "raw 3-centre integrals");
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(dV, metric, sizeof(double) * na * na, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dA, eri3, sizeof(double) * na * c->npair, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = nopt::cholesky_f64(dV, na, (int)na, c->info, c->stream);
+    int info = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&info, c->info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      rc = set_error(NOPT_EHIP, std::string("metric factorisation: ") + hip_msg(e));
+    } else if (info != 0) {
+      rc = set_error(NOPT_ECHOL, "Coulomb metric is not positive definite (pivot " +
+                                     std::to_string(info - 1) + ")");
+    } else {
+      e = nopt::trsm_lower_f64(dV, na, (int)na, dA, c->npair, c->npair, c->stream);
+      if (e != hipSuccess) rc = set_error(NOPT_EHIP, std::string("triangular solve: ") + hip_msg(e));
+    }
+  }
+  if (!rc) rc = finish_B(c, dA);
+  dev_free(dV);
+  dev_free(dA);
+  return rc;
+}
+
+int nopt_jk_set_B_fitted(nopt_jk_ctx* c, const double* Bp) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_set_B_fitted: null input");
+  double* dA = nullptr;
+  rc = dev_alloc((void**)&dA, sizeof(double) * c->naux * c->npair, "packed B");
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(dA, Bp, sizeof(double) * c->naux * c->npair, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) rc = set_error(NOPT_EHIP, std::string("copy B: ") + hip_msg(e));
+  }
+  if (!rc) rc = finish_B(c, dA);
+  dev_free(dA);
+  return rc;
+}
+
+int nopt_jk_get_B(nopt_jk_ctx* c, double* Bp) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_get_B: null output");
+  double* dA = nullptr;
+  const long total = c->naux * c->npair;
+  rc = dev_alloc((void**)&dA, sizeof(double) * total, "packed B");
+  if (rc) return rc;
+  k_pack_B<<<(unsigned)((total + 255) / 256), 256, 0, c->stream>>>(c->B64, c->ld64, (int)c->nbf, c->npair,
+                                                                   c->naux, dA);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(Bp, dA, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dev_free(dA);
+  if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("nopt_jk_get_B: ") + hip_msg(e));
+  return NOPT_OK;
+}
+
+long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
+  if (!c) return 0;
+  long b = 0;
+  if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
+  b += (long)c->X_bytes + (long)c->Cp_bytes;
+  b += sizeof(double) * (c->naux * c->mtiles + c->naux + (long)c->nsj * c->nbf * c->nbf +
+                         (long)c->nsk * c->ntiles * KT * KT + c->nbf * c->ld64 + 5 * c->nbf * c->nbf);
+  return b;
+}
+
+int nopt_jk_build_device(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, double* dJ,
+                         double* dK, int prec, void* stream) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (nocc < 0 || nocc > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return enqueue_build(c, dD, dC, nocc, dJ, dK, prec, s);
+}
+
+int nopt_jk_build(nopt_jk_ctx* c, const double* D, const double* C, long nocc, double* J, double* K,
+                  int prec) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (nocc < 0 || nocc > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
+  const long nn = c->nbf * c->nbf;
+  if (J && !D) return set_error(NOPT_EINVAL, "J requested without a density");
+  if (K && nocc > 0 && !C) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
+  if (D) HIPCHK(hipMemcpyAsync(c->dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  if (C && nocc > 0) {
+    rc = upload(c, &c->dC, &c->dC_cap, C, nocc * c->nbf);
+    if (rc) return rc;
+  }
+  rc = enqueue_build(c, D ? c->dD : nullptr, (C && nocc > 0) ? c->dC : nullptr, nocc, J ? c->dJ : nullptr,
+                     K ? c->dK : nullptr, prec, c->stream);
+  if (rc) return rc;
+  if (J) HIPCHK(hipMemcpyAsync(J, c->dJ, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+  if (K) {
+    if (nocc == 0)
+      std::memset(K, 0, sizeof(double) * nn);
+    else
+      HIPCHK(hipMemcpyAsync(K, c->dK, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return NOPT_OK;
+}
+
+int nopt_jk_fock_energy(nopt_jk_ctx* c, const double* H, const double* D, const double* J,
+                        const double* K, double cj, double ck, double* F, double* E) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!H || !D) return set_error(NOPT_EINVAL, "nopt_jk_fock_energy: H and D are required");
+  const long nn = c->nbf * c->nbf;
+  HIPCHK(hipMemcpyAsync(c->dH, H, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  if (J) HIPCHK(hipMemcpyAsync(c->dJ, J, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  if (K) HIPCHK(hipMemcpyAsync(c->dK, K, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  const int nblk = (int)std::min<long>(1024, (nn + 255) / 256);
+  k_fock_energy<<<nblk, 256, 0, c->stream>>>(c->dH, c->dD, J ? c->dJ : nullptr, K ? c->dK : nullptr, cj, ck,
+                                             nn, c->dF, c->epart);
+  k_sum<<<1, 256, 0, c->stream>>>(c->epart, nblk, c->dE);
+  HIPCHK(hipGetLastError());
+  if (F) HIPCHK(hipMemcpyAsync(F, c->dF, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+  double e = 0.0;
+  HIPCHK(hipMemcpyAsync(&e, c->dE, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (E) *E = e;
+  return NOPT_OK;
+}
+
+int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, long nocc, int prec,
+                            double* E, double* G) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!H || !C || nocc <= 0 || nocc > c->nbf)
+    return set_error(NOPT_EINVAL, "nopt_jk_energy_gradient: H, C and 0 < nocc <= nbf are required");
+  const long nbf = c->nbf, nn = nbf * nbf;
+  rc = upload(c, &c->dC, &c->dC_cap, C, nocc * nbf);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(c->dH, H, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  k_density<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(c->dC, (int)nocc, (int)nbf, c->dD);
+  HIPCHK(hipGetLastError());
+  rc = enqueue_build(c, c->dD, c->dC, nocc, c->dJ, c->dK, prec, c->stream);
+  if (rc) return rc;
+  const int nblk = (int)std::min<long>(1024, (nn + 255) / 256);
+  k_fock_energy<<<nblk, 256, 0, c->stream>>>(c->dH, c->dD, c->dJ, c->dK, 2.0, 1.0, nn, c->dF, c->epart);
+  k_sum<<<1, 256, 0, c->stream>>>(c->epart, nblk, c->dE);
+  HIPCHK(hipGetLastError());
+  if (G) {
+    if (!c->dG) {
+      rc = dev_alloc((void**)&c->dG, sizeof(double) * nn, "gradient");
+      if (rc) return rc;
+    }
+    // G (nocc x nbf) = 4 C F
+    HIPCHK(nopt::gemm_f64(false, (int)nocc, (int)nbf, (int)nbf, 4.0, c->dC, nbf, c->dF, nbf, 0.0, c->dG, nbf,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(G, c->dG, sizeof(double) * nocc * nbf, hipMemcpyDeviceToHost, c->stream));
+  }
+  double e = 0.0;
+  HIPCHK(hipMemcpyAsync(&e, c->dE, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (E) *E = e;
+  return NOPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,189 @@
+"""Host-side interface to the RI-J/K build on the GPU (include/nopt_jk.h).
+
+``JKContext`` owns one device context with the fitted B tensor resident in
+HBM. Its methods mirror NOPT's RI layer (include/RI.h:6-23):
+
+  set_B(eri3, metric)        gen_RI_AA (src/RI.cpp:34-79), metric folded in
+  build(D, C, nocc)          DM_to_F_transform_RI (src/RI.cpp:115-176)
+  fock_energy(H, D, J, K)    molecule::calc_F_AO (src/molecule.cpp:4407-4438)
+                             and the RHF energy (src/SCF.cpp:134)
+  energy_gradient(H, C)      E_RHF(C) and dE/dC = 4 C F (the orbital gradient
+                             g_ia = 4 F_ai of src/converger_2_1.cpp:54-62)
+
+Orbital factors follow NOPT: rows are orbitals (C[i, m]); densities are
+normalised to 1 (D = C^T C over the occupied rows, src/SCF.cpp:63-73).
+There is no CPU fallback: every call runs the HIP kernels of libnoptjk.so and
+fails loudly when the library or the GPU is missing.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class JKContext:
+    def __init__(self, nbf, naux, device=0):
+        lib = L.lib()
+        if not hasattr(lib, "nopt_jk_create"):
+            raise ImportError("libnoptjk.so was built without the HIP kernels")
+        h = ctypes.c_void_p()
+        L.check(lib.nopt_jk_create(int(device), int(nbf), int(naux), ctypes.byref(h)))
+        self._h = h
+        self.nbf, self.naux = int(nbf), int(naux)
+        self.npair = self.nbf * (self.nbf + 1) // 2
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().nopt_jk_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _arr(self, a, shape):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"expected shape {shape}, got {a.shape}")
+        return a
+
+    def set_B(self, eri3_packed, metric):
+        A = self._arr(eri3_packed, (self.naux, self.npair))
+        V = self._arr(metric, (self.naux, self.naux))
+        L.check(L.lib().nopt_jk_set_B(self._h, L.dptr(A), L.dptr(V)))
+
+    def set_B_fitted(self, B_packed):
+        B = self._arr(B_packed, (self.naux, self.npair))
+        L.check(L.lib().nopt_jk_set_B_fitted(self._h, L.dptr(B)))
+
+    def get_B(self):
+        out = np.empty((self.naux, self.npair))
+        L.check(L.lib().nopt_jk_get_B(self._h, L.dptr(out)))
+        return out
+
+    def device_bytes(self):
+        return int(L.lib().nopt_jk_device_bytes(self._h))
+
+    def build(self, D=None, C=None, nocc=None, prec="fp64", J=True, K=True):
+        """(J, K) for density D (nbf x nbf) and the first nocc rows of C."""
+        n = self.nbf
+        p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+        Dm = self._arr(D, (n, n)) if D is not None else None
+        Cm = None
+        if C is not None:
+            Cm = np.ascontiguousarray(C, dtype=np.float64)
+            if Cm.ndim != 2 or Cm.shape[1] != n:
+                raise ValueError("C must be (norb, nbf) with orbitals as rows")
+            if nocc is None:
+                nocc = Cm.shape[0]
+            if nocc > Cm.shape[0]:
+                raise ValueError("nocc exceeds the rows of C")
+        nocc = int(nocc or 0)
+        Jo = np.empty((n, n)) if J else None
+        Ko = np.empty((n, n)) if K else None
+        lib = L.lib()
+        L.check(lib.nopt_jk_build(self._h, L.dptr(Dm) if Dm is not None else None,
+                                  L.dptr(Cm) if Cm is not None else None, nocc,
+                                  L.dptr(Jo) if Jo is not None else None,
+                                  L.dptr(Ko) if Ko is not None else None, p))
+        return Jo, Ko
+
+    def build_device(self, dD, dC, nocc, dJ, dK, prec="fp64", stream=0):
+        """Device-pointer build (integer addresses), enqueued on `stream`."""
+        p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+        L.check(L.lib().nopt_jk_build_device(self._h, dD, dC, int(nocc), dJ, dK, p, stream or None))
+
+    def set_timing(self, enable=True):
+        L.check(L.lib().nopt_jk_set_timing(self._h, 1 if enable else 0))
+
+    PHASES = ("copy_in", "half_transform", "coulomb", "exchange", "finalize")
+
+    def read_timing(self):
+        """{phase: summed ms}, builds recorded since the last read."""
+        ms = np.zeros(5)
+        nb = ctypes.c_long()
+        L.check(L.lib().nopt_jk_read_timing(self._h, L.dptr(ms), ctypes.byref(nb)))
+        return dict(zip(self.PHASES, ms.tolist())), nb.value
+
+    def stream(self):
+        return L.lib().nopt_jk_stream(self._h)
+
+    def fock_energy(self, H, D, J=None, K=None, cj=2.0, ck=1.0):
+        n = self.nbf
+        Hm, Dm = self._arr(H, (n, n)), self._arr(D, (n, n))
+        Jm = self._arr(J, (n, n)) if J is not None else None
+        Km = self._arr(K, (n, n)) if K is not None else None
+        F = np.empty((n, n))
+        E = ctypes.c_double()
+        L.check(L.lib().nopt_jk_fock_energy(self._h, L.dptr(Hm), L.dptr(Dm),
+                                            L.dptr(Jm) if Jm is not None else None,
+                                            L.dptr(Km) if Km is not None else None,
+                                            float(cj), float(ck), L.dptr(F), ctypes.byref(E)))
+        return F, E.value
+
+    def energy_gradient(self, H, C, nocc=None, prec="fp64"):
+        """E = sum D(H+F) (no V_nuc) and G = dE/dC = 4 C F (nocc x nbf)."""
+        n = self.nbf
+        Hm = self._arr(H, (n, n))
+        Cm = np.ascontiguousarray(C, dtype=np.float64)
+        nocc = int(nocc or Cm.shape[0])
+        G = np.empty((nocc, n))
+        E = ctypes.c_double()
+        p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+        L.check(L.lib().nopt_jk_energy_gradient(self._h, L.dptr(Hm), L.dptr(Cm), nocc, p,
+                                                ctypes.byref(E), L.dptr(G)))
+        return E.value, G
+
+
+class DeviceArray:
+    """FP64 buffer in device memory (plumbing for device-pointer builds)."""
+
+    def __init__(self, shape, device=0):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.device = device
+        self.ptr = L.lib().nopt_dev_malloc(device, max(self.nbytes, 8))
+        if not self.ptr:
+            raise MemoryError(L.lib().nopt_last_error().decode())
+
+    @classmethod
+    def from_host(cls, a, device=0):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape, device)
+        L.check(L.lib().nopt_dev_memcpy(d.ptr, a.ctypes.data, d.nbytes, 0))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape)
+        L.check(L.lib().nopt_dev_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            L.lib().nopt_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def synchronize(device=0):
+    L.check(L.lib().nopt_dev_synchronize(device))
+
+
+def context_for(mol, device=0):
+    """Context with B formed on the device from the molecule's integrals."""
+    ctx = JKContext(mol.nbf, mol.naux, device)
+    ctx.set_B(mol.eri3_packed(), mol.metric())
+    return ctx

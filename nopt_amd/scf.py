@@ -1,0 +1,90 @@
+"""Closed-shell RI-RHF driver around the GPU RI-J/K build.
+
+The loop mirrors NOPT's RHF (src/SCF.cpp:78-223): per iteration the density
+D = C_occ C_occ^T (gen_HF_DM, src/SCF.cpp:63-73), one RI-J/K build and
+F = H + 2J - K (calc_F_AO, src/molecule.cpp:4407-4438), the energy
+E = sum D.F + sum D.H + V_nuc (src/SCF.cpp:134) and the orbital gradient
+4 F_ai (src/converger_2_1.cpp:54-62). NOPT steps with SOSCF/L-BFGS from a
+Hueckel or SAD guess; this driver uses a core-Hamiltonian guess and DIIS on
+FC S - S C F... (the commutator FDS - SDF), which converges to the same fixed
+point. Orthogonalisation (S^-1/2), diagonalisation and the energy stay FP64.
+
+Mixed precision as iterative refinement: builds use `prec` (FP32 or BF16x3
+exchange) until the DIIS error falls below `switch_err`, then FP64 for the
+remaining iterations, so the converged energy is that of the FP64 build.
+"""
+import time
+
+import numpy as np
+
+from .jk import JKContext
+
+
+class SCFResult(dict):
+    __getattr__ = dict.__getitem__
+
+
+def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, max_it=100,
+        diis_size=8, verbose=False, H=None, S=None):
+    """Returns SCFResult(E, C (nbf x nbf, columns are MOs), eps, D, F, iters,
+    builds {prec: count}, history)."""
+    own = ctx is None
+    if own:
+        ctx = JKContext(mol.nbf, mol.naux)
+        ctx.set_B(mol.eri3_packed(), mol.metric())
+    try:
+        if H is None or S is None:
+            S_, T_, V_ = mol.one_electron()
+            S = S_ if S is None else S
+            H = (T_ + V_) if H is None else H
+        s, U = np.linalg.eigh(S)
+        X = U @ np.diag(s ** -0.5) @ U.T
+        eps, Cp = np.linalg.eigh(X.T @ H @ X)
+        C = X @ Cp
+        nocc = mol.nocc
+        cur = prec
+        E_old, D_old = 0.0, None
+        fs, es = [], []
+        builds, hist = {}, []
+        for it in range(max_it):
+            Co = C[:, :nocc]
+            D = Co @ Co.T
+            J, K = ctx.build(D, Co.T, nocc, prec=cur)
+            builds[cur] = builds.get(cur, 0) + 1
+            F, e_el = ctx.fock_energy(H, D, J, K, 2.0, 1.0)
+            E = e_el + mol.e_nuc
+            err = F @ D @ S - S @ D @ F
+            emax = float(np.abs(err).max())
+            drms = float(np.sqrt(np.mean((D - D_old) ** 2))) if D_old is not None else np.inf
+            hist.append((it, cur, E, emax, drms))
+            if verbose:
+                print(f"ITER {it} {cur:7s} E={E:.12f} dE={E - E_old:.3e} err={emax:.3e} drms={drms:.3e}")
+            done = abs(E - E_old) < e_conv and drms < d_conv
+            if done and cur == "fp64":
+                break
+            if cur != "fp64" and (emax < switch_err or done):
+                cur = "fp64"  # FP64 end-game: refine with exact exchange
+                fs, es = [], []
+            E_old, D_old = E, D
+            fs.append(F)
+            es.append(err)
+            if len(fs) > diis_size:
+                fs.pop(0)
+                es.pop(0)
+            n = len(fs)
+            Bm = -np.ones((n + 1, n + 1))
+            Bm[-1, -1] = 0.0
+            for i in range(n):
+                for j in range(i, n):
+                    Bm[i, j] = Bm[j, i] = np.sum(es[i] * es[j])
+            rhs = np.zeros(n + 1)
+            rhs[-1] = -1.0
+            cf = np.linalg.lstsq(Bm, rhs, rcond=None)[0][:n]
+            Fd = sum(c * f for c, f in zip(cf, fs))
+            eps, Cp = np.linalg.eigh(X.T @ Fd @ X)
+            C = X @ Cp
+        return SCFResult(E=E, C=C, eps=eps, D=D, F=F, H=H, S=S, iters=it, builds=builds, history=hist,
+                         converged=done)
+    finally:
+        if own:
+            ctx.close()
