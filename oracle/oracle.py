@@ -332,10 +332,11 @@ def rhf(mol, e_conv=1e-11, d_conv=1e-9, max_it=100, verbose=False):
 
 def brute_force_jk(eri3, V, D):
     """Explicit four-index RI contraction, tiny systems only:
-    (mn|rs)_RI = sum_PQ (mn|P) V^-1_PQ (Q|rs); J = (mn|rs) D_rs,
-    K_mn = sum_rs (mr|ns) D_rs."""
-    Vi = np.linalg.inv(V)
-    g4 = np.einsum("mnp,pq,rsq->mnrs", eri3, Vi, eri3)
+    (mn|rs)_RI = sum_PQ (mn|P) V^-1_PQ (Q|rs), formed as W^T W with
+    W = L^-1 (P|mn) (V = L L^T); J = (mn|rs) D_rs, K_mn = sum_rs (mr|ns) D_rs."""
+    n = eri3.shape[0]
+    W = np.linalg.solve(np.linalg.cholesky(V), eri3.reshape(n * n, -1).T)
+    g4 = (W.T @ W).reshape(n, n, n, n)
     J = np.einsum("mnrs,rs->mn", g4, D)
     K = np.einsum("mrns,rs->mn", g4, D)
     return J, K

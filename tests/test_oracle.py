@@ -1,0 +1,166 @@
+"""Pins of the oracle itself (CPU): the RI-RHF energies NOPT prints in its
+example logs, closed forms of the Boys function and of s-type integrals, and
+the factored NOPT J/K chain against the explicit four-index RI contraction."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.special as sp
+
+import oracle as O
+from nopt_amd import geometry as G
+from systems import SYSTEMS, GOLDEN_E
+
+BASIS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nopt_amd", "basis")
+
+
+def gbs(name):
+    return os.path.join(BASIS, name + ".gbs")
+
+
+# ------------------------------------------------------------------ Boys ---
+@pytest.mark.parametrize("n", [0, 1, 3, 7, 12])
+@pytest.mark.parametrize("T", [0.0, 1e-3, 0.7, 5.0, 17.3, 34.9, 35.1, 60.0, 300.0])
+def test_boys_against_confluent_hypergeometric(n, T):
+    # F_n(T) = 1F1(n+1/2; n+3/2; -T) / (2n+1)
+    ref = sp.hyp1f1(n + 0.5, n + 1.5, -T) / (2 * n + 1)
+    assert abs(O.lib().oracle_boys(n, T) - ref) <= 1e-13 * max(ref, 1e-300) + 1e-300
+
+
+def test_boys_zero_order_closed_form():
+    for T in (0.3, 2.0, 11.0, 40.0):
+        ref = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        assert abs(O.lib().oracle_boys(0, T) - ref) < 1e-15
+
+
+# ------------------------------------------------- s-type closed forms -----
+def _one_shell_basis(alphas, centers, l=0):
+    lib = {"X": [(l, [a], [1.0]) for a in alphas]}
+    sh = O.Shells(lib, ["X"], [centers[0]])
+    return sh
+
+
+def test_s_overlap_kinetic_nuclear_closed_forms():
+    a, b = 0.8, 1.7
+    A = np.array([0.0, 0.0, 0.0])
+    Bc = np.array([0.3, -0.4, 1.1])
+    lib = {"H": [(0, [a], [1.0])], "He": [(0, [b], [1.0])]}
+    sh = O.Shells(lib, ["H", "He"], [A, Bc])
+    n = 2
+    S, T, V = (np.zeros((n, n)) for _ in range(3))
+    Z = np.array([1.0, 0.0])
+    xyz = np.ascontiguousarray(np.concatenate([A, Bc]))
+    O.lib().oracle_one_electron(*sh.args(), 2, O._d(Z), O._d(xyz), O._d(S), O._d(T), O._d(V))
+    p, mu = a + b, a * b / (a + b)
+    R2 = float(np.sum((A - Bc) ** 2))
+    Na, Nb = (2 * a / math.pi) ** 0.75, (2 * b / math.pi) ** 0.75
+    S_ab = Na * Nb * (math.pi / p) ** 1.5 * math.exp(-mu * R2)
+    assert abs(S[0, 1] - S_ab) < 1e-15 and abs(S[0, 0] - 1) < 1e-15
+    T_ab = mu * (3 - 2 * mu * R2) * S_ab
+    assert abs(T[0, 1] - T_ab) < 1e-14
+    assert abs(T[0, 0] - 1.5 * a) < 1e-14
+    # s at the nucleus: <1/r> = 2 sqrt(2a/pi)
+    assert abs(V[0, 0] + 2 * math.sqrt(2 * a / math.pi)) < 1e-14
+    P = (a * A + b * Bc) / p
+    V_ab = -2 * math.pi / p * Na * Nb * math.exp(-mu * R2) * O.lib().oracle_boys(0, p * float(np.sum(P ** 2)))
+    assert abs(V[0, 1] - V_ab) < 1e-14
+
+
+def test_s_coulomb_two_and_three_centre_closed_forms():
+    a, b, c = 1.3, 0.45, 0.9
+    A, Bc, C = np.zeros(3), np.array([0.0, 0.8, 0.2]), np.array([1.0, -0.3, 0.5])
+    orb = O.Shells({"H": [(0, [a], [1.0])], "He": [(0, [b], [1.0])]}, ["H", "He"], [A, Bc])
+    aux = O.Shells({"H": [(0, [c], [1.0])], "He": [(0, [a], [1.0])]}, ["H", "He"], [C, A])
+    tf = O.harmonic_tf(0)
+    V = np.zeros((2, 2))
+    O.lib().oracle_metric(*aux.args(), O._d(tf), O._d(V))
+    N = lambda e: (2 * e / math.pi) ** 0.75
+    rho = c * a / (c + a)
+    ref = 2 * math.pi ** 2.5 / (c * a * math.sqrt(c + a)) * N(c) * N(a) * \
+        O.lib().oracle_boys(0, rho * float(np.sum((C - A) ** 2)))
+    assert abs(V[0, 1] - ref) < 1e-14
+    out = np.zeros((2, 2, 2))
+    O.lib().oracle_eri3(*orb.args(), *aux.args(), O._d(tf), O._d(out))
+    p, mu = a + b, a * b / (a + b)
+    P = (a * A + b * Bc) / p
+    rho = p * c / (p + c)
+    ref3 = 2 * math.pi ** 2.5 / (p * c * math.sqrt(p + c)) * N(a) * N(b) * N(c) * \
+        math.exp(-mu * float(np.sum((A - Bc) ** 2))) * O.lib().oracle_boys(0, rho * float(np.sum((P - C) ** 2)))
+    assert abs(out[0, 1, 0] - ref3) < 1e-14 and abs(out[1, 0, 0] - ref3) < 1e-14
+
+
+def test_harmonic_basis_is_traceless_and_normalised():
+    for l in range(2, 5):
+        H = O.harmonic_basis(l)
+        assert H.shape == (2 * l + 1, (l + 1) * (l + 2) // 2)
+        mon = O.monomials(l)
+        # Laplacian of each row vanishes
+        low = {m: k for k, m in enumerate(O.monomials(l - 2))}
+        for row in H:
+            lap = np.zeros(len(low))
+            for k, (x, y, z) in enumerate(mon):
+                for d, e in enumerate((x, y, z)):
+                    if e >= 2:
+                        t = [x, y, z]
+                        t[d] -= 2
+                        lap[low[tuple(t)]] += e * (e - 1) * row[k]
+            assert np.abs(lap).max() < 1e-12
+
+
+def test_translation_invariance_of_rhf_energy():
+    sym, xyz = G.water()
+    e = []
+    for shift in (np.zeros(3), np.array([1.7, -2.3, 0.4])):
+        m = O.OracleMolecule.from_angstrom(sym, np.asarray(xyz) + shift, gbs("cc-pvdz"), gbs("cc-pvdz-jkfit"))
+        e.append(O.rhf(m)["E"])
+    assert abs(e[0] - e[1]) < 1e-10
+
+
+# ------------------------------------------------------- NOPT log pins -----
+@pytest.mark.parametrize("name", ["C2H4", "C6H8", "guanine"])
+def test_oracle_reproduces_nopt_log_energies(name):
+    sym, xyz = SYSTEMS[name]
+    m = O.OracleMolecule.from_angstrom(sym, xyz, gbs("cc-pvdz"), gbs("cc-pvdz-ri"))
+    r = O.rhf(m, e_conv=1e-11, d_conv=1e-9)
+    assert r["dropped"] == 0
+    assert abs(r["E"] - GOLDEN_E[name]) < 3e-10, (r["E"], GOLDEN_E[name])
+
+
+def test_planted_sign_error_in_exchange_breaks_the_pin():
+    """The golden energy distinguishes a plausible mistake: K with the wrong
+    sign in the Fock matrix moves the C2H4 energy far from NOPT's value."""
+    sym, xyz = SYSTEMS["C2H4"]
+    m = O.OracleMolecule.from_angstrom(sym, xyz, gbs("cc-pvdz"), gbs("cc-pvdz-ri"))
+    r = O.rhf(m)
+    D, H, J, K = r["D"], r["H"], r["J"], r["K"]
+    good = np.sum(D * (2 * H + 2 * J - K)) + m.e_nuc
+    bad = np.sum(D * (2 * H + 2 * J + K)) + m.e_nuc
+    assert abs(good - GOLDEN_E["C2H4"]) < 3e-10
+    assert abs(bad - GOLDEN_E["C2H4"]) > 1.0
+
+
+# --------------------------------------------- factored vs explicit RI -----
+def test_nopt_jk_chain_matches_explicit_four_index_contraction():
+    sym, xyz = G.water()
+    m = O.OracleMolecule.from_angstrom(sym, xyz, gbs("cc-pvdz"), gbs("cc-pvdz-jkfit"))
+    r = O.rhf(m)
+    eri3 = m.eri3_nopt()
+    Jb, Kb = O.brute_force_jk(eri3, m.metric(), r["D"])
+    J, K = r["jk"](r["D"], r["C"].T, m.nocc)
+    assert np.abs(J - Jb).max() < 1e-11
+    assert np.abs(K - Kb).max() < 1e-11
+
+
+def test_jk_invariants():
+    sym, xyz = G.water()
+    m = O.OracleMolecule.from_angstrom(sym, xyz, gbs("cc-pvdz"), gbs("cc-pvdz-jkfit"))
+    r = O.rhf(m)
+    D, C = r["D"], r["C"]
+    J, K = r["jk"](D, C.T, m.nocc)
+    assert np.abs(K - K.T).max() < 1e-12 and np.abs(J - J.T).max() < 1e-12
+    assert np.linalg.eigvalsh(K).min() > -1e-10  # PSD for a PSD density
+    U, _ = np.linalg.qr(np.random.default_rng(1).standard_normal((m.nocc, m.nocc)))
+    J2, K2 = r["jk"](D, (C @ U).T, m.nocc)  # rotation among occupied orbitals
+    assert np.abs(K2 - K).max() < 1e-11 and np.abs(J2 - J).max() < 1e-12
+    assert np.sum(D * J) > 0 and np.sum(D * K) > 0
