@@ -6,10 +6,12 @@ Workload: all-trans C20H42, cc-pVDZ (490 spherical functions) / cc-pVDZ-JKFIT
 density, K from the occupied orbital factor) of the converged RI-RHF density,
 with B = L^-1 (P|mn) resident in HBM and the inputs already on the device.
 
-The headline mode (default --prec fp32) runs the half-transform and exchange
-as FP32 MFMA products with FP64 accumulation and J in FP64; its energy error
-against the FP64 build on the same density is reported and must stay below
-1 micro-Hartree. The FP64 build is timed beside it.
+Every precision mode is timed (J is FP64 in all of them): fp64, fp32 (FP32
+MFMA products, FP64 accumulation), bf16x6 (three-piece bf16 split of FP32
+operands, six MFMA products, FP64 accumulation) and bf16x3. The headline
+value is the fastest mode whose energy error against the FP64 build on the
+same density stays below 1 micro-Hartree (--prec auto), next to the FP64
+build's own rate.
 
 The CPU baseline times the oracle's restatement of NOPT's
 DM_to_F_transform_RI (Cartesian AOs, raw (mn|P), V^-1/2 applied per build,
@@ -62,7 +64,8 @@ def roofline(phase, ms, nbf, naux, nocc, prec):
     t = ms * 1e-3
     if phase == "half_transform":
         xsz = 8 if prec == "fp64" else 4
-        byts = naux * nbf * nbf * 8 + naux * nbf * nocc * xsz + nbf * nbf * 8 + nocc * nbf * xsz
+        bsz = 5 if prec == "int8" else 8  # five int8 digit planes, or FP64
+        byts = naux * nbf * nbf * bsz + naux * nbf * nocc * xsz + nbf * nbf * 8 + nocc * nbf * xsz
         flops = 2.0 * naux * nbf * nbf * nocc
         if prec == "fp64":
             return dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK["fp64_TF"], unit="TFLOP/s",
@@ -70,10 +73,12 @@ def roofline(phase, ms, nbf, naux, nocc, prec):
                         kernel="k_half<double>", algorithmic_flops=flops)
         return dict(bound="hbm", achieved=byts / t / 1e9, peak=PEAK["hbm_GBps"], unit="GB/s",
                     frac=byts / t / 1e9 / PEAK["hbm_GBps"], traffic=None,
-                    kernel="k_half<float>" if prec == "fp32" else "k_half_bf16x3", algorithmic_bytes=byts)
+                    kernel={"fp32": "k_half<float>", "int8": "k_half_i8", "bf16x6": "k_half_bf16<3>",
+                            "bf16x3": "k_half_bf16<2>"}[prec], algorithmic_bytes=byts)
     if phase == "exchange":
         flops = float(naux * nocc * nbf * (nbf + 1))
-        pk = PEAK["fp64_TF"] if prec == "fp64" else PEAK["fp32_TF"] if prec == "fp32" else PEAK["bf16_TF"] / 3
+        pk = {"fp64": PEAK["fp64_TF"], "fp32": PEAK["fp32_TF"], "int8": PEAK["fp32_TF"],
+              "bf16x6": PEAK["bf16_TF"] / 6, "bf16x3": PEAK["bf16_TF"] / 3}[prec]
         return dict(bound="mfma", achieved=flops / t / 1e12, peak=pk, unit="TFLOP/s", frac=flops / t / 1e12 / pk,
                     traffic=None, kernel="k_syrk", algorithmic_flops=flops)
     if phase == "coulomb":
@@ -123,7 +128,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3", "fp64"])
+    ap.add_argument("--prec", default="auto", choices=["auto", "int8", "fp32", "bf16x6", "bf16x3", "fp64"],
+                    help="headline mode; auto = fastest mode with |dE| < 1 uEh against FP64")
     ap.add_argument("--alkane", type=int, default=20, help="carbon count of the all-trans alkane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -165,9 +171,10 @@ def main():
     dJ, dK = DeviceArray((nbf, nbf)), DeviceArray((nbf, nbf))
 
     # accuracy of each mode against the FP64 build on the same density
+    modes = ("fp64", "int8", "fp32", "bf16x6", "bf16x3")
     acc = {}
     ref = None
-    for p in ("fp64", args.prec, "bf16x3"):
+    for p in modes:
         ctx.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, p)
         synchronize()
         J, K = dJ.to_host(), dK.to_host()
@@ -177,19 +184,22 @@ def main():
                       max_abs_dK=float(np.abs(K - ref[1]).max()))
     log(f"[bench] accuracy vs FP64: {json.dumps(acc)}")
 
-    rate, ms, phases = time_builds(ctx, dD, dC, nocc, dJ, dK, args.prec, args.steps, args.warmup, synchronize,
-                                   timing=True)
-    log(f"[bench] {args.prec}: {rate:.1f} builds/s ({ms:.3f} ms/build); phases {phases}")
-    rate64, ms64, ph64 = time_builds(ctx, dD, dC, nocc, dJ, dK, "fp64", max(10, args.steps // 4), 2, synchronize,
-                                     timing=True)
-    log(f"[bench] fp64: {rate64:.1f} builds/s ({ms64:.3f} ms/build); phases {ph64}")
-    rate_bf, ms_bf, ph_bf = time_builds(ctx, dD, dC, nocc, dJ, dK, "bf16x3", max(10, args.steps // 4), 2,
-                                        synchronize, timing=True)
-    log(f"[bench] bf16x3: {rate_bf:.1f} builds/s ({ms_bf:.3f} ms/build); phases {ph_bf}")
+    runs = {}
+    for p in modes:
+        runs[p] = time_builds(ctx, dD, dC, nocc, dJ, dK, p, args.steps, args.warmup, synchronize, timing=True)
+        log(f"[bench] {p}: {runs[p][0]:.1f} builds/s ({runs[p][1]:.3f} ms/build); phases {runs[p][2]}")
+    if args.prec == "auto":
+        ok = [p for p in modes if p != "fp64" and abs(acc[p]["dE"]) < 1e-6]
+        prec = max(ok, key=lambda p: runs[p][0]) if ok else "fp64"
+    else:
+        prec = args.prec
+    rate, ms, phases = runs[prec]
+    rate64 = runs["fp64"][0]
 
     dom = max(("half_transform", "coulomb", "exchange"), key=lambda k: phases[k])
-    roof = roofline(dom, phases[dom], nbf, naux, nocc, args.prec)
-    others = {k: roofline(k, phases[k], nbf, naux, nocc, args.prec) for k in ("half_transform", "coulomb", "exchange")}
+    roof = roofline(dom, phases[dom], nbf, naux, nocc, prec)
+    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p) for k in ("half_transform", "coulomb", "exchange")}
+            for p in modes}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -212,23 +222,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32 (fp64 acc)", "bf16x3": "bf16x3 (fp64 acc)", "fp64": "f64"}[args.prec],
+        "dtype": {"int8": "int8 digits, exact int32 acc (fp64 combine)", "fp32": "f32 (fp64 acc)",
+                  "bf16x6": "bf16x6 (fp32 split, fp64 acc)",
+                  "bf16x3": "bf16x3 (fp64 acc)", "fp64": "f64"}[prec],
         "data": "C20H42 geometry built in-repo; converged FP64 RI-RHF density and orbitals",
         "config": {
             "workload": f"RI-J/K build, all-trans C{args.alkane}H{2 * args.alkane + 2} cc-pVDZ/cc-pVDZ-JKFIT, "
                         f"nbf {nbf}, naux {naux}, nocc {nocc}, B FP64 resident in HBM",
-            "precision": args.prec,
-            "dE_vs_fp64_Eh": acc[args.prec]["dE"],
-            "max_abs_dJ": acc[args.prec]["max_abs_dJ"],
-            "max_abs_dK": acc[args.prec]["max_abs_dK"],
+            "precision": prec,
+            "dE_vs_fp64_Eh": acc[prec]["dE"],
+            "max_abs_dJ": acc[prec]["max_abs_dJ"],
+            "max_abs_dK": acc[prec]["max_abs_dK"],
             "fp64_builds_per_s": round(rate64, 2),
             "speedup_vs_fp64": round(rate / rate64, 3),
-            "bf16x3_builds_per_s": round(rate_bf, 2),
-            "bf16x3_dE_vs_fp64_Eh": acc["bf16x3"]["dE"],
-            "phase_ms": {k: round(v, 4) for k, v in phases.items()},
-            "phase_ms_fp64": {k: round(v, 4) for k, v in ph64.items()},
-            "kernels": {k: (None if v is None else {kk: (round(vv, 4) if isinstance(vv, float) else vv)
-                                                    for kk, vv in v.items()}) for k, v in others.items()},
+            "modes": {p: {"builds_per_s": round(runs[p][0], 2), "dE_vs_fp64_Eh": acc[p]["dE"],
+                          "max_abs_dK": acc[p]["max_abs_dK"],
+                          "phase_ms": {k: round(v, 4) for k, v in runs[p][2].items()},
+                          "kernels": {k: (None if v is None else {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                                                                  for kk, vv in v.items()})
+                                      for k, v in kern[p].items()}} for p in modes},
             "E_rhf_fp64": res.E,
             "setup_s": {"host_integrals": round(t_ints, 1), "B_device": round(t_B, 2), "scf": round(t_scf, 1)},
         },

@@ -37,8 +37,10 @@ extern "C" {
 /* Arithmetic of the half-transform and exchange contraction. J, the metric
  * and the Fock assembly are FP64 in every mode. */
 #define NOPT_PREC_FP64 0   /* FP64 MFMA */
-#define NOPT_PREC_FP32 1   /* FP32 MFMA, FP32 accumulation */
-#define NOPT_PREC_BF16X3 2 /* bf16 hi/lo split, three MFMA products, FP32 accumulation */
+#define NOPT_PREC_FP32 1   /* FP32 MFMA in short chains, FP64 accumulation */
+#define NOPT_PREC_BF16X3 2 /* bf16 hi/lo split, three MFMA products (about 16-bit products) */
+#define NOPT_PREC_BF16X6 3 /* three-piece bf16 split of FP32, six MFMA products */
+#define NOPT_PREC_INT8 4   /* int8 digit split with power-of-two row scales, exact int32 MFMA accumulation */
 
 typedef struct nopt_jk_ctx nopt_jk_ctx;
 

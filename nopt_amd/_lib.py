@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnoptjk.so")
 BASIS_DIR = os.path.join(HERE, "basis")
 
 NOPT_OK, NOPT_EINVAL, NOPT_ENOMEM, NOPT_ECHOL, NOPT_EHIP, NOPT_EIO, NOPT_ESTATE = range(7)
-PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3, "int8": 4}
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)

@@ -12,7 +12,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnoptjk.so")
 
 HOST_SRCS = ["host/basis.cpp", "host/ints.cpp", "host/errors.cpp", "host/mol_capi.cpp"]
-HIP_SRCS = ["kernels/linalg.hip", "kernels/jk.hip"]
+HIP_SRCS = ["kernels/linalg.hip", "kernels/bf16.hip", "kernels/ozaki.hip", "kernels/jk.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")

@@ -31,7 +31,9 @@
 #include "../../../include/nopt_jk.h"
 #include "../host/errors.hpp"
 #include "common.hpp"
+#include "bf16.hpp"
 #include "linalg.hpp"
+#include "ozaki.hpp"
 
 using nopt::set_error;
 
@@ -39,7 +41,7 @@ namespace {
 
 constexpr int BM = 128;     // half-transform rows per tile
 constexpr int KT = 128;     // exchange tile edge
-constexpr int NSPLIT_J_MIN = 8;  // minimum Q-split of the Coulomb pass
+constexpr int NSPLIT_J_MIN = 16; // minimum Q-split of the Coulomb pass
 constexpr int JCHUNK_MAX = 4096; // gamma values cached in LDS per split
 
 // ------------------------------------------------------------------ copies
@@ -202,7 +204,8 @@ __global__ __launch_bounds__(256) void k_half(const double* __restrict__ B, long
         }
     }
     if constexpr (sizeof(T) == 4) {
-      // FP32 chains of BK terms, accumulated across chains in FP64
+      // FP32 chains of BK terms (rounded to nearest by the MFMA), accumulated
+      // across chains in FP64
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -235,145 +238,6 @@ __global__ __launch_bounds__(256) void k_half(const double* __restrict__ B, long
   }
 }
 
-// ------------------------------------------------------ half-transform bf16x3
-// B (FP64) and C (FP32) are split into bf16 hi/lo while they are staged in
-// LDS; each 16x16x32 step issues lo*hi, hi*lo and hi*hi products into one FP32
-// accumulator. X is written in FP32. ld is a multiple of 16; the 32-wide
-// k-steps mask the tail.
-template <int NT>
-__global__ __launch_bounds__(256) void k_half_bf16x3(const double* __restrict__ B, long ld, int nbf,
-                                                     const float* __restrict__ Cp,
-                                                     const double* __restrict__ Dp,
-                                                     float* __restrict__ X, long ldx,
-                                                     double* __restrict__ gpart, int mtiles) {
-  constexpr int BN = 16 * NT, BK = 32, PK = BK + 8;
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[BM][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Al[BM][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Ch[BN][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Cl[BN][PK];
-  __shared__ double red[4];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int mtile = blockIdx.x;
-  const long Q = blockIdx.y;
-  const int cg = blockIdx.z;
-  const int m0 = mtile * BM;
-  const int ar = t >> 1, ak = (t & 1) * 16;
-  const bool arow_ok = (m0 + ar) < nbf;
-  const double* Arow = B + ((long)Q * nbf + m0 + ar) * ld + ak;
-  const double* Drow = Dp + (long)(m0 + ar) * ld + ak;
-  const bool c_ok = t < 2 * BN;
-  const float* Crow = Cp + ((long)cg * BN + (t >> 1)) * ld + ak;
-  f4_t acc[2][NT];
-  d4_t acc64[2][NT];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      acc[i][j] = f4_t{0, 0, 0, 0};
-      acc64[i][j] = d4_t{0, 0, 0, 0};
-    }
-  double rd = 0.0;
-  double a_reg[16], d_reg[16];
-  float c_reg[16];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const bool kin = k0 + ak + 8 * h + 8 <= ld;
-      if (arow_ok && kin) {
-        load8<double>(Arow + k0 + 8 * h, a_reg + 8 * h);
-        load8<double>(Drow + k0 + 8 * h, d_reg + 8 * h);
-      } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          a_reg[8 * h + i] = 0.0;
-          d_reg[8 * h + i] = 0.0;
-        }
-      }
-      if (c_ok) {
-        if (kin)
-          load8<float>(Crow + k0 + 8 * h, c_reg + 8 * h);
-        else
-#pragma unroll
-          for (int i = 0; i < 8; ++i) c_reg[8 * h + i] = 0.f;
-      }
-    }
-  };
-  gload(0);
-  for (int k0 = 0; k0 < ld; k0 += BK) {
-    __syncthreads();
-    {
-      uint16_t h[16], l[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        split_bf16((float)a_reg[i], h[i], l[i]);
-        rd += a_reg[i] * d_reg[i];
-      }
-      uint4* ph = reinterpret_cast<uint4*>(&Ah[ar][ak]);
-      uint4* pl = reinterpret_cast<uint4*>(&Al[ar][ak]);
-      ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
-      ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
-      pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
-      pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
-    }
-    if (c_ok) {
-      uint16_t h[16], l[16];
-#pragma unroll
-      for (int i = 0; i < 16; ++i) split_bf16(c_reg[i], h[i], l[i]);
-      uint4* ph = reinterpret_cast<uint4*>(&Ch[t >> 1][ak]);
-      uint4* pl = reinterpret_cast<uint4*>(&Cl[t >> 1][ak]);
-      ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
-      ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
-      pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
-      pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
-    }
-    __syncthreads();
-    if (k0 + BK < ld) gload(k0 + BK);
-    uint4 ah[2], al[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int r = w * 32 + i * 16 + lane % 16;
-      ah[i] = *reinterpret_cast<const uint4*>(&Ah[r][8 * (lane / 16)]);
-      al[i] = *reinterpret_cast<const uint4*>(&Al[r][8 * (lane / 16)]);
-    }
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const int c = j * 16 + lane % 16;
-      const uint4 bh = *reinterpret_cast<const uint4*>(&Ch[c][8 * (lane / 16)]);
-      const uint4 bl = *reinterpret_cast<const uint4*>(&Cl[c][8 * (lane / 16)]);
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
-        acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
-        acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
-      }
-    }
-    // FP32 chain of one 32-wide k-step, accumulated in FP64
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < NT; ++j) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
-        acc[i][j] = f4_t{0, 0, 0, 0};
-      }
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int m = m0 + w * 32 + i * 16 + 4 * (lane / 16) + v;
-      if (m < nbf) {
-        float* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
-#pragma unroll
-        for (int j = 0; j < NT; ++j) xr[j * 16] = (float)acc64[i][j][v];
-      }
-    }
-  if (cg == 0) {
-    double s = block_sum256(rd, red);
-    if (t == 0) gpart[Q * mtiles + mtile] = s;
-  }
-}
-
 // ------------------------------------------------------------------ gamma
 __global__ void k_gamma(const double* __restrict__ gpart, int mtiles, int naux,
                         double* __restrict__ gamma) {
@@ -386,47 +250,49 @@ __global__ void k_gamma(const double* __restrict__ gpart, int mtiles, int naux,
 
 // ---------------------------------------------------------------- Coulomb
 // Jpart[s][m][n] = sum_{Q in split s} gamma_Q B[Q][m][n] over the lower
-// triangle. A block owns the row pair (p, nbf-1-p), nbf+1 elements.
-template <typename T>
-__global__ __launch_bounds__(256) void k_jpart(const T* __restrict__ B, long ld, int nbf, int naux,
+// triangle. A block owns the row pair (p, nbf-1-p): nbf+1 elements, read as
+// aligned pairs of columns (a row's last pair may include its first upper
+// element, which lies inside the row because ld > nbf and is not written).
+// The Q loop is unrolled so that UNR loads per thread are in flight.
+__global__ __launch_bounds__(256) void k_jpart(const double* __restrict__ B, long ld, int nbf, int naux,
                                                const double* __restrict__ gamma,
                                                double* __restrict__ Jpart) {
-  constexpr int EPT = 8;  // elements per thread: nbf + 1 <= 2048
+  constexpr int UNR = 8;
   __shared__ double g[JCHUNK_MAX];
   const int p = blockIdx.x, s = blockIdx.y, ns = gridDim.y;
   const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
   for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) g[q - q0] = gamma[q];
   __syncthreads();
   const int m1 = p, m2 = nbf - 1 - p;
-  const int n1 = m1 + 1, cnt = (m2 != m1) ? n1 + m2 + 1 : n1;
-  int row[EPT], col[EPT];
-  double acc[EPT];
-#pragma unroll
-  for (int j = 0; j < EPT; ++j) {
-    const int e = threadIdx.x + j * 256;
-    acc[j] = 0.0;
-    if (e < n1) {
-      row[j] = m1;
-      col[j] = e;
-    } else if (e < cnt) {
-      row[j] = m2;
-      col[j] = e - n1;
-    } else {
-      row[j] = -1;
-      col[j] = 0;
-    }
-  }
-  for (int q = q0; q < q1; ++q) {
-    const double gq = g[q - q0];
-    const T* Bq = B + (long)q * nbf * ld;
-#pragma unroll
-    for (int j = 0; j < EPT; ++j)
-      if (row[j] >= 0) acc[j] += gq * (double)Bq[(long)row[j] * ld + col[j]];
-  }
+  const int s1 = (m1 + 2) / 2, s2 = (m2 != m1) ? (m2 + 2) / 2 : 0;
+  const long qstride = (long)nbf * ld;
   double* Js = Jpart + (long)s * nbf * nbf;
+  for (int slot = threadIdx.x; slot < s1 + s2; slot += blockDim.x) {
+    const int row = slot < s1 ? m1 : m2;
+    const int col = 2 * (slot < s1 ? slot : slot - s1);
+    const double* bp = B + (long)q0 * qstride + (long)row * ld + col;
+    double a0 = 0.0, a1 = 0.0;
+    int q = q0;
+    for (; q + UNR <= q1; q += UNR) {
+      double2 v[UNR];
 #pragma unroll
-  for (int j = 0; j < EPT; ++j)
-    if (row[j] >= 0) Js[(long)row[j] * nbf + col[j]] = acc[j];
+      for (int u = 0; u < UNR; ++u) v[u] = *reinterpret_cast<const double2*>(bp + (long)u * qstride);
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        a0 += g[q - q0 + u] * v[u].x;
+        a1 += g[q - q0 + u] * v[u].y;
+      }
+      bp += (long)UNR * qstride;
+    }
+    for (; q < q1; ++q) {
+      const double2 v = *reinterpret_cast<const double2*>(bp);
+      a0 += g[q - q0] * v.x;
+      a1 += g[q - q0] * v.y;
+      bp += qstride;
+    }
+    Js[(long)row * nbf + col] = a0;
+    if (col + 1 <= row) Js[(long)row * nbf + col + 1] = a1;
+  }
 }
 
 // --------------------------------------------------------------- exchange
@@ -539,119 +405,6 @@ __global__ __launch_bounds__(256) void k_syrk(const T* __restrict__ X, long ldx,
       }
 }
 
-// bf16x3 exchange: X (FP32) is split into hi/lo while staged in LDS.
-__global__ __launch_bounds__(256) void k_syrk_bf16x3(const float* __restrict__ X, long ldx, int nbf,
-                                                     int naux, int kpad,
-                                                     double* __restrict__ Kpart) {
-  constexpr int BK = 32, PK = BK + 8;
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[KT][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Al[KT][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bh[KT][PK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bl[KT][PK];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = w >> 1, wc = w & 1;
-  int bi, bj;
-  tri_tile(blockIdx.x, bi, bj);
-  const int s = blockIdx.y, ns = gridDim.y;
-  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
-  const int r = t >> 1, kk = (t & 1) * 16;
-  const bool ra = bi * KT + r < nbf, rb = bj * KT + r < nbf;
-  f4_t acc[4][4];
-  d4_t acc64[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      acc[i][j] = f4_t{0, 0, 0, 0};
-      acc64[i][j] = d4_t{0, 0, 0, 0};
-    }
-  float a_reg[16], b_reg[16];
-  auto gload = [&](int q, int i0) {
-    const float* xa = X + ((long)q * nbf + bi * KT + r) * ldx + i0 + kk;
-    const float* xb = X + ((long)q * nbf + bj * KT + r) * ldx + i0 + kk;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int base = i0 + kk + 8 * h;
-      const bool inr = base + 8 <= ldx;  // ldx is a multiple of 16
-      if (ra && inr)
-        load8<float>(xa + 8 * h, a_reg + 8 * h);
-      else
-#pragma unroll
-        for (int i = 0; i < 8; ++i) a_reg[8 * h + i] = 0.f;
-      if (rb && inr)
-        load8<float>(xb + 8 * h, b_reg + 8 * h);
-      else
-#pragma unroll
-        for (int i = 0; i < 8; ++i) b_reg[8 * h + i] = 0.f;
-    }
-  };
-  auto stage = [&](uint16_t (*H)[PK], uint16_t (*L)[PK], const float* v) {
-    uint16_t h[16], l[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) split_bf16(v[i], h[i], l[i]);
-    uint4* ph = reinterpret_cast<uint4*>(&H[r][kk]);
-    uint4* pl = reinterpret_cast<uint4*>(&L[r][kk]);
-    ph[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h));
-    ph[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(h + 8));
-    pl[0] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l));
-    pl[1] = __builtin_bit_cast(uint4, *reinterpret_cast<uint16_t(*)[8]>(l + 8));
-  };
-  if (q0 < q1) gload(q0, 0);
-  for (int q = q0; q < q1; ++q) {
-    for (int i0 = 0; i0 < kpad; i0 += BK) {
-      __syncthreads();
-      stage(Ah, Al, a_reg);
-      stage(Bh, Bl, b_reg);
-      __syncthreads();
-      int nq = q, ni = i0 + BK;
-      if (ni >= kpad) {
-        ni = 0;
-        ++nq;
-      }
-      if (nq < q1) gload(nq, ni);
-      uint4 ah[4], al[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int rr = wr * 64 + i * 16 + lane % 16;
-        ah[i] = *reinterpret_cast<const uint4*>(&Ah[rr][8 * (lane / 16)]);
-        al[i] = *reinterpret_cast<const uint4*>(&Al[rr][8 * (lane / 16)]);
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int cc = wc * 64 + j * 16 + lane % 16;
-        const uint4 bh = *reinterpret_cast<const uint4*>(&Bh[cc][8 * (lane / 16)]);
-        const uint4 bl = *reinterpret_cast<const uint4*>(&Bl[cc][8 * (lane / 16)]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          acc[i][j] = mfma_bf16(al[i], bh, acc[i][j]);
-          acc[i][j] = mfma_bf16(ah[i], bl, acc[i][j]);
-          acc[i][j] = mfma_bf16(ah[i], bh, acc[i][j]);
-        }
-      }
-      // FP32 chain of one 32-wide k-step, accumulated in FP64
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc64[i][j][v] += (double)acc[i][j][v];
-          acc[i][j] = f4_t{0, 0, 0, 0};
-        }
-    }
-  }
-  double* Kt = Kpart + ((long)s * gridDim.x + blockIdx.x) * KT * KT;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int rr = wr * 64 + i * 16 + 4 * (lane / 16) + v;
-        const int cc = wc * 64 + j * 16 + lane % 16;
-        Kt[rr * KT + cc] = acc64[i][j][v];
-      }
-}
-
 // --------------------------------------------------------------- finalize
 // J = sum_s Jpart[s], K = sum_s Kpart[s] over the lower triangle, written to
 // both triangles of J, K (ld nbf). Either output may be null.
@@ -739,6 +492,16 @@ struct nopt_jk_ctx {
          *dF = nullptr, *dG = nullptr, *epart = nullptr, *dE = nullptr;
   long dC_cap = 0;
   int* info = nullptr;
+  // int8 digit planes of B (5 x naux*nbf x ld8) with per-row exponents, and
+  // the per-build digit planes of the orbital factor (4 x ldx x ld8)
+  long ld8 = 0;
+  int mtiles64 = 0;
+  int8_t* Bd = nullptr;
+  int* Be = nullptr;
+  void* Cd = nullptr;
+  size_t Cd_bytes = 0;
+  void* Ce = nullptr;
+  size_t Ce_bytes = 0;
   // optional per-phase timing: events for up to kTimedBuilds builds
   bool timing = false;
   int timed = 0;
@@ -817,26 +580,6 @@ hipError_t launch_half(int nt, dim3 grid, hipStream_t s, const double* B, long l
   return hipGetLastError();
 }
 
-hipError_t launch_half_bf16x3(int nt, dim3 grid, hipStream_t s, const double* B, long ld, int nbf,
-                              const float* Cp, const double* Dp, float* X, long ldx, double* gpart,
-                              int mtiles) {
-#define NOPT_HALF(NTV) \
-  case NTV:            \
-    k_half_bf16x3<NTV><<<grid, 256, 0, s>>>(B, ld, nbf, Cp, Dp, X, ldx, gpart, mtiles); \
-    break;
-  switch (nt) {
-    NOPT_HALF(1)
-    NOPT_HALF(2)
-    NOPT_HALF(4)
-    NOPT_HALF(6)
-    NOPT_HALF(8)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef NOPT_HALF
-  return hipGetLastError();
-}
-
 int check_ctx(nopt_jk_ctx* c, bool need_B) {
   if (!c) return set_error(NOPT_EINVAL, "null context");
   if (need_B && !c->B64) return set_error(NOPT_ESTATE, "B tensor not set (call nopt_jk_set_B first)");
@@ -856,6 +599,13 @@ int finish_B(nopt_jk_ctx* c, const double* dBp) {
   k_unpack_B<<<(unsigned)((total + 255) / 256), 256, 0, c->stream>>>(dBp, c->npair, (int)c->nbf, rows,
                                                                      c->B64, c->ld64);
   HIPCHK(hipGetLastError());
+  // five int8 digit planes per row of B for the integer-split builds
+  if (!c->Bd) {
+    int rc = dev_alloc((void**)&c->Bd, (size_t)5 * rows * c->ld8, "B digit planes");
+    if (!rc) rc = dev_alloc((void**)&c->Be, sizeof(int) * rows, "B row exponents");
+    if (rc) return rc;
+  }
+  HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)c->nbf, rows, 5, c->Bd, c->ld8, rows, c->Be, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return NOPT_OK;
 }
@@ -866,7 +616,7 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   const int nbf = (int)c->nbf, naux = (int)c->naux;
   const bool wantK = dK != nullptr && nocc > 0;
   const bool wantJ = dJ != nullptr;
-  if (prec < 0 || prec > 2) return set_error(NOPT_EINVAL, "unknown precision mode");
+  if (prec < 0 || prec > 4) return set_error(NOPT_EINVAL, "unknown precision mode");
   if (wantK && !dC) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
   if (wantJ && !dD) return set_error(NOPT_EINVAL, "J requested without a density");
   if (!wantJ && !wantK) return NOPT_OK;
@@ -874,7 +624,9 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   const int nt = nt_for(wantK ? nocc : 1, &ngroups);
   const long ldx = (long)nt * 16 * ngroups;  // padded orbital count
   const bool f64 = prec == NOPT_PREC_FP64;
-  const long ldb = c->ld64;
+  const bool i8 = prec == NOPT_PREC_INT8;
+  const long ldb = i8 ? c->ld8 : c->ld64;  // leading dimension of the padded D and C copies
+  const int mt = i8 ? c->mtiles64 : c->mtiles;
   const size_t esz = f64 ? sizeof(double) : sizeof(float);
   int rc;
   size_t cap = c->X_bytes;
@@ -885,6 +637,14 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
   if (rc) return rc;
   c->Cp_bytes = cpb;
+  if (i8) {
+    size_t b1 = c->Cd_bytes, b2 = c->Ce_bytes;
+    rc = ensure(&c->Cd, &b1, (size_t)4 * ldx * c->ld8, "orbital digit planes");
+    if (!rc) rc = ensure(&c->Ce, &b2, sizeof(int) * ldx, "orbital exponents");
+    if (rc) return rc;
+    c->Cd_bytes = b1;
+    c->Ce_bytes = b2;
+  }
   const bool tm = c->timing && c->timed < kTimedBuilds;
   auto mark = [&](int ph) {
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
@@ -899,7 +659,9 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
       k_pad_copy<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c->Dp, ldb, 0, 0, c->Dp, ldb, nbf);
     const long nc = ldx * ldb;
     const int rows = wantK ? (int)nocc : 0;
-    if (f64)
+    if (i8)
+      HIPCHK(nopt::split_rows_i8(dC ? dC : c->Dp, nbf, nbf, rows, 4, (int8_t*)c->Cd, c->ld8, ldx, (int*)c->Ce, s));
+    else if (f64)
       k_pad_copy<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(dC ? dC : c->Dp, nbf, rows, nbf,
                                                                       (double*)c->Cp, ldb, (int)ldx);
     else
@@ -909,35 +671,40 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   }
   // half-transform (+ gamma partials)
   mark(1);
-  dim3 gh(c->mtiles, naux, wantK ? ngroups : 1);
+  dim3 gh(mt, naux, wantK ? ngroups : 1);
   hipError_t e;
-  if (prec == NOPT_PREC_FP64)
+  if (i8)
+    e = nopt::launch_half_i8(nt, gh, s, c->Bd, c->naux * c->nbf * c->ld8, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
+                             ldx * c->ld8, (const int*)c->Ce, c->Dp, ldb, (float*)c->X, ldx, c->gpart, mt);
+  else if (prec == NOPT_PREC_FP64)
     e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X,
                             ldx, c->gpart, c->mtiles);
   else if (prec == NOPT_PREC_FP32)
     e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
                            c->gpart, c->mtiles);
   else
-    e = launch_half_bf16x3(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
-                           c->gpart, c->mtiles);
+    e = nopt::launch_half_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, nt, gh, s, c->B64, ldb, nbf,
+                               (const float*)c->Cp, c->Dp, (float*)c->X, ldx, c->gpart, c->mtiles);
   HIPCHK(e);
   mark(2);
   if (wantJ) {
-    k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, c->mtiles, naux, c->gamma);
+    k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, mt, naux, c->gamma);
     dim3 gj((nbf + 1) / 2, c->nsj);
-    k_jpart<double><<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, c->Jpart);
+    k_jpart<<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, c->Jpart);
     HIPCHK(hipGetLastError());
   }
   mark(3);
   if (wantK) {
-    const int kpad = (int)round_up(nocc, prec == NOPT_PREC_BF16X3 ? 32 : 16);
+    const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
+    const int kpad = (int)round_up(nocc, bf ? 32 : 16);
     dim3 gk(c->ntiles, c->nsk);
     if (prec == NOPT_PREC_FP64)
       k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X, ldx, nbf, naux, kpad, c->Kpart);
-    else if (prec == NOPT_PREC_FP32)
+    else if (prec == NOPT_PREC_FP32 || i8)
       k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
     else
-      k_syrk_bf16x3<<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
+      HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)c->X, ldx, nbf,
+                                    naux, kpad, c->Kpart));
     HIPCHK(hipGetLastError());
   }
   mark(4);
@@ -983,8 +750,10 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   c->nbf = nbf;
   c->naux = naux;
   c->npair = nbf * (nbf + 1) / 2;
-  c->ld64 = round_up(nbf, 16);
+  c->ld64 = round_up(nbf + 1, 16);  // ld > nbf: see k_jpart
   c->mtiles = (int)((nbf + BM - 1) / BM);
+  c->ld8 = round_up(nbf, 64);  // int8 MFMA k-steps of 64
+  c->mtiles64 = (int)((nbf + 63) / 64);
   c->ktn = (int)((nbf + KT - 1) / KT);
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
   c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
@@ -997,11 +766,11 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
     return set_error(NOPT_EHIP, std::string("nopt_jk_create: ") + hip_msg(e));
   }
   const long nn = nbf * nbf;
-  if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles, "gamma partials");
+  if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles64, "gamma partials");
   if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * naux, "gamma");
   if (!rc) rc = dev_alloc((void**)&c->Jpart, sizeof(double) * c->nsj * nn, "J partials");
   if (!rc) rc = dev_alloc((void**)&c->Kpart, sizeof(double) * (long)c->nsk * c->ntiles * KT * KT, "K partials");
-  if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * nbf * c->ld64, "padded density");
+  if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * nbf * std::max(c->ld64, c->ld8), "padded density");
   if (!rc) rc = dev_alloc((void**)&c->dJ, sizeof(double) * nn, "J");
   if (!rc) rc = dev_alloc((void**)&c->dK, sizeof(double) * nn, "K");
   if (!rc) rc = dev_alloc((void**)&c->dD, sizeof(double) * nn, "D");
@@ -1040,6 +809,10 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->epart);
   dev_free(c->dE);
   dev_free(c->info);
+  dev_free(c->Bd);
+  dev_free(c->Be);
+  dev_free(c->Cd);
+  dev_free(c->Ce);
   for (auto& ev : c->tev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1184,9 +957,11 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   if (!c) return 0;
   long b = 0;
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
-  b += (long)c->X_bytes + (long)c->Cp_bytes;
-  b += sizeof(double) * (c->naux * c->mtiles + c->naux + (long)c->nsj * c->nbf * c->nbf +
-                         (long)c->nsk * c->ntiles * KT * KT + c->nbf * c->ld64 + 5 * c->nbf * c->nbf);
+  if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf;
+  b += (long)c->X_bytes + (long)c->Cp_bytes + (long)c->Cd_bytes + (long)c->Ce_bytes;
+  b += sizeof(double) * (c->naux * c->mtiles64 + c->naux + (long)c->nsj * c->nbf * c->nbf +
+                         (long)c->nsk * c->ntiles * KT * KT + c->nbf * std::max(c->ld64, c->ld8) +
+                         5 * c->nbf * c->nbf);
   return b;
 }
 

@@ -87,7 +87,7 @@ def test_metric_not_positive_definite_is_reported(h2o_case):
         assert ei.value.code == NOPT_ECHOL
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x3", 2e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 5e-6), ("bf16x6", 5e-6), ("bf16x3", 2e-4), ("int8", 5e-8)])
 def test_mixed_precision_close_to_fp64(c2h4_case, prec, tol):
     om, r, pm = c2h4_case
     D, Cc = r["D"], r["C"]
