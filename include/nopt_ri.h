@@ -1,0 +1,64 @@
+// Drop-in replacement for the Fock-build part of NOPT's RI layer
+// (include/RI.h:6-23), backed by the GPU context of nopt_jk.h.
+//
+// The functions keep NOPT's names, argument lists and semantics, with C linkage
+// (NOPT declares them as C++ functions), so an SCF or CASSCF loop written
+// against NOPT's API compiles against this header and links to libnoptjk.so:
+//   - D is normalised to 1 and MO rows are orbitals; K uses the first n_c rows
+//     (DM_to_F_transform_RI, src/RI.cpp:115-176); J and K are overwritten.
+//   - DM_to_F_transform_RI returns 1 on success like NOPT's; the others return
+//     0. Where NOPT would exit or throw, these return a negative NOPT_E* code
+//     and leave the message in nopt_last_error().
+//   - One global context, like NOPT's module globals (src/RI.cpp:16-32); calls
+//     come from one host thread.
+//
+// NOPT's gen_RI_AA(molecule*) computes (P|Q) and (mn|P) with libint2 and then
+// hands them over with RI_gpu_attach (the integral layer stays on the host);
+// standalone callers can use RI_gpu_attach_molecule with the integral engine
+// of nopt_ints.h instead.
+#ifndef NOPT_RI_H
+#define NOPT_RI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Replaces the tail of gen_RI_AA (src/RI.cpp:55-72): AUX_V is the Coulomb
+// metric (aux_n_ao x aux_n_ao, RI_V_calc) and RI_AA the raw 3-centre tensor in
+// NOPT's layout RI_AA[(m*n_ao+n)*aux_n_ao+P] (calc_2el_AO_INTS_RI). The metric
+// is factored and folded into B on the device; neither array is kept.
+int RI_gpu_attach(const double* AUX_V, const double* RI_AA, long n_ao, long aux_n_ao, int device);
+
+// Standalone set-up: integrals from the host engine of nopt_ints.h (coordinates
+// in bohr, Cartesian orbital basis as NOPT reads it when pure == 0).
+int RI_gpu_attach_molecule(int natom, const int* Z, const double* xyz_bohr, const char* basis_gbs,
+                           const char* aux_gbs, int pure, int device);
+
+// Arithmetic of the half-transform and exchange for the following builds
+// (a NOPT_PREC_* value of nopt_jk.h; FP64 by default).
+int RI_gpu_set_precision(int prec);
+
+// NOPT's RI_core_realloc (src/RI.cpp:104-113): the device buffers grow on
+// demand, so this only checks that a context exists.
+int RI_core_realloc(long n_c, long n_ao);
+
+// NOPT's regen_RI_AA (src/RI.cpp:81-101) minus the rebuild: releases the
+// device context so that the next RI_gpu_attach forms B for a new geometry.
+int regen_RI_AA_release();
+
+// NOPT's DM_to_F_transform_RI (src/RI.cpp:115-176).
+int DM_to_F_transform_RI(double* J, double* K, double* DM, double* MO, long n_c, long n_ao);
+
+// The inactive J/K half of NOPT's calc_2el_MO_INTS_RI (src/RI.cpp:229-290):
+// J from DM_D and K from the first n_c rows of MO_VEC; returns 0 without
+// touching J and K when either is NULL, like NOPT. The active-space
+// integrals (tu|vw) that NOPT also forms are not produced (MO_INTS, MO_VEC1,
+// MO_VEC2, n_a and t are accepted and ignored).
+int calc_2el_MO_INTS_RI(long n_ao, double* DM_D, double* J, double* K, double* MO_INTS, double* MO_VEC,
+                        double* MO_VEC1, double* MO_VEC2, long n_c, long n_a, long t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

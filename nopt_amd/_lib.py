@@ -62,6 +62,18 @@ GPU_SYMBOLS = [
     ("nopt_dev_free", None, [_vp]),
     ("nopt_dev_memcpy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     ("nopt_dev_synchronize", ctypes.c_int, [ctypes.c_int]),
+    # NOPT-named RI entry points (include/nopt_ri.h)
+    ("RI_gpu_attach", ctypes.c_int, [_c_double_p, _c_double_p, ctypes.c_long, ctypes.c_long, ctypes.c_int]),
+    ("RI_gpu_attach_molecule", ctypes.c_int, [ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_char_p,
+                                              ctypes.c_char_p, ctypes.c_int, ctypes.c_int]),
+    ("RI_gpu_set_precision", ctypes.c_int, [ctypes.c_int]),
+    ("RI_core_realloc", ctypes.c_int, [ctypes.c_long, ctypes.c_long]),
+    ("regen_RI_AA_release", ctypes.c_int, []),
+    ("DM_to_F_transform_RI", ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, _c_double_p,
+                                            ctypes.c_long, ctypes.c_long]),
+    ("calc_2el_MO_INTS_RI", ctypes.c_int, [ctypes.c_long, _c_double_p, _c_double_p, _c_double_p, _c_double_p,
+                                           _c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_long,
+                                           ctypes.c_long]),
 ]
 
 

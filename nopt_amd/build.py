@@ -12,6 +12,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnoptjk.so")
 
 HOST_SRCS = ["host/basis.cpp", "host/ints.cpp", "host/errors.cpp", "host/mol_capi.cpp"]
+# host code that calls the GPU entry points (left out of a --host-only build)
+GLUE_SRCS = ["host/ri_shim.cpp"]
 HIP_SRCS = ["kernels/linalg.hip", "kernels/bf16.hip", "kernels/ozaki.hip", "kernels/jk.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -35,7 +37,7 @@ def build(hip=True, verbose=True):
     """Compile every source (always; no staleness check) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for s in HOST_SRCS:
+    for s in HOST_SRCS + (GLUE_SRCS if hip else []):
         o = os.path.join(BUILD, s.replace("/", "_") + ".o")
         _run(["g++"] + CXXFLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)

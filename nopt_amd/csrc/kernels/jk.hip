@@ -409,8 +409,9 @@ __global__ __launch_bounds__(256) void k_syrk(const T* __restrict__ X, long ldx,
 // J = sum_s Jpart[s], K = sum_s Kpart[s] over the lower triangle, written to
 // both triangles of J, K (ld nbf). Either output may be null.
 __global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const double* __restrict__ Kpart,
-                           int nsk, int ntiles, int nbf, double* __restrict__ J,
+                           int nsk, int ntiles, int kt, int nbf, double* __restrict__ J,
                            double* __restrict__ K) {
+  const int KT = kt;  // exchange tile edge of the partials
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)nbf * nbf) return;
   const int m = (int)(e / nbf), n = (int)(e % nbf);
@@ -502,6 +503,11 @@ struct nopt_jk_ctx {
   size_t Cd_bytes = 0;
   void* Ce = nullptr;
   size_t Ce_bytes = 0;
+  void* Xd = nullptr;  // 4 digit planes of X (naux*nbf rows x ldk)
+  size_t Xd_bytes = 0;
+  void* Xe = nullptr;
+  size_t Xe_bytes = 0;
+  int ntiles8 = 0, nsk8 = 0;  // 64x64 exchange tiles of the integer path
   // optional per-phase timing: events for up to kTimedBuilds builds
   bool timing = false;
   int timed = 0;
@@ -627,7 +633,7 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   const bool i8 = prec == NOPT_PREC_INT8;
   const long ldb = i8 ? c->ld8 : c->ld64;  // leading dimension of the padded D and C copies
   const int mt = i8 ? c->mtiles64 : c->mtiles;
-  const size_t esz = f64 ? sizeof(double) : sizeof(float);
+  const size_t esz = (f64 || i8) ? sizeof(double) : sizeof(float);
   int rc;
   size_t cap = c->X_bytes;
   rc = ensure(&c->X, &cap, esz * c->naux * c->nbf * ldx, "half-transformed tensor X");
@@ -673,9 +679,11 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   mark(1);
   dim3 gh(mt, naux, wantK ? ngroups : 1);
   hipError_t e;
+  const long bplane = c->naux * c->nbf * c->ld8;
   if (i8)
-    e = nopt::launch_half_i8(nt, gh, s, c->Bd, c->naux * c->nbf * c->ld8, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
-                             ldx * c->ld8, (const int*)c->Ce, c->Dp, ldb, (float*)c->X, ldx, c->gpart, mt);
+    e = wantK ? nopt::launch_half_i8(nt, gh, s, c->Bd, bplane, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
+                                     ldx * c->ld8, (const int*)c->Ce, (double*)c->X, ldx)
+              : hipSuccess;
   else if (prec == NOPT_PREC_FP64)
     e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X,
                             ldx, c->gpart, c->mtiles);
@@ -688,7 +696,10 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   HIPCHK(e);
   mark(2);
   if (wantJ) {
-    k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, mt, naux, c->gamma);
+    if (i8)
+      HIPCHK(nopt::launch_gamma_i8(naux, s, c->Bd, bplane, c->Be, c->ld8, nbf, c->Dp, ldb, c->gamma));
+    else
+      k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, mt, naux, c->gamma);
     dim3 gj((nbf + 1) / 2, c->nsj);
     k_jpart<<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, c->Jpart);
     HIPCHK(hipGetLastError());
@@ -698,9 +709,22 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
     const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
     const int kpad = (int)round_up(nocc, bf ? 32 : 16);
     dim3 gk(c->ntiles, c->nsk);
-    if (prec == NOPT_PREC_FP64)
+    if (i8) {
+      // four digit planes per row (Q, m) of X, then the integer exchange
+      const long ldk = round_up(nocc, 64), rows = c->naux * c->nbf;
+      size_t b1 = c->Xd_bytes, b2 = c->Xe_bytes;
+      rc = ensure(&c->Xd, &b1, (size_t)4 * rows * ldk, "exchange digit planes");
+      if (!rc) rc = ensure(&c->Xe, &b2, sizeof(int) * rows, "exchange row exponents");
+      if (rc) return rc;
+      c->Xd_bytes = b1;
+      c->Xe_bytes = b2;
+      HIPCHK(nopt::split_rows_i8((const double*)c->X, ldx, (int)nocc, rows, 4, (int8_t*)c->Xd, ldk, rows,
+                                 (int*)c->Xe, s));
+      HIPCHK(nopt::launch_syrk_i8(dim3(c->ntiles8, c->nsk8), s, (const int8_t*)c->Xd, rows * ldk,
+                                  (const int*)c->Xe, ldk, nbf, naux, c->Kpart));
+    } else if (prec == NOPT_PREC_FP64)
       k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X, ldx, nbf, naux, kpad, c->Kpart);
-    else if (prec == NOPT_PREC_FP32 || i8)
+    else if (prec == NOPT_PREC_FP32)
       k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
     else
       HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)c->X, ldx, nbf,
@@ -709,9 +733,9 @@ int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc,
   }
   mark(4);
   const long nn = (long)nbf * nbf;
-  k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(c->Jpart, c->nsj, c->Kpart, c->nsk,
-                                                          c->ntiles, nbf, wantJ ? dJ : nullptr,
-                                                          wantK ? dK : nullptr);
+  k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(
+      c->Jpart, c->nsj, c->Kpart, i8 ? c->nsk8 : c->nsk, i8 ? c->ntiles8 : c->ntiles,
+      i8 ? nopt::kSyrkI8Tile : KT, nbf, wantJ ? dJ : nullptr, wantK ? dK : nullptr);
   HIPCHK(hipGetLastError());
   mark(5);
   if (tm) ++c->timed;
@@ -757,6 +781,11 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   c->ktn = (int)((nbf + KT - 1) / KT);
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
   c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
+  {
+    const int kt8 = (int)((nbf + nopt::kSyrkI8Tile - 1) / nopt::kSyrkI8Tile);
+    c->ntiles8 = kt8 * (kt8 + 1) / 2;
+    c->nsk8 = std::max(1, std::min((int)naux, (2048 + c->ntiles8 - 1) / c->ntiles8));
+  }
   c->nsj = std::max(NSPLIT_J_MIN, (int)((naux + JCHUNK_MAX - 1) / JCHUNK_MAX));
   int rc = NOPT_OK;
   hipError_t e = hipSetDevice(device);
@@ -769,7 +798,11 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles64, "gamma partials");
   if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * naux, "gamma");
   if (!rc) rc = dev_alloc((void**)&c->Jpart, sizeof(double) * c->nsj * nn, "J partials");
-  if (!rc) rc = dev_alloc((void**)&c->Kpart, sizeof(double) * (long)c->nsk * c->ntiles * KT * KT, "K partials");
+  if (!rc)
+    rc = dev_alloc((void**)&c->Kpart,
+                   sizeof(double) * std::max((long)c->nsk * c->ntiles * KT * KT,
+                                             (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile),
+                   "K partials");
   if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * nbf * std::max(c->ld64, c->ld8), "padded density");
   if (!rc) rc = dev_alloc((void**)&c->dJ, sizeof(double) * nn, "J");
   if (!rc) rc = dev_alloc((void**)&c->dK, sizeof(double) * nn, "K");
@@ -813,6 +846,8 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->Be);
   dev_free(c->Cd);
   dev_free(c->Ce);
+  dev_free(c->Xd);
+  dev_free(c->Xe);
   for (auto& ev : c->tev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -1,19 +1,23 @@
-// Integer-split (Ozaki-type) half-transform on the int8 matrix cores.
+// Integer-split (Ozaki-type) kernels on the int8 matrix cores.
 //
 // Every row r of an operand is scaled by a power of two, 2^-e_r with
 // e_r = exponent of the row's largest magnitude, and the scaled row
 // v = x 2^-e_r (|v| < 1) is written as signed base-128 digits
-//   v = 2^-6 d_0 + 2^-13 d_1 + 2^-20 d_2 + ... ,  d_k = rint(...) in [-64, 64],
+//   x 2^(6-e_r) = d_0 + 2^-7 d_1 + 2^-14 d_2 + ... ,  d_k = rint(...) in [-64, 64],
 // each digit rounded to nearest, so the truncation error is unbiased and
 // below 2^-(6+7S) of the row maximum for S digits. A product of two split
-// operands is the weighted sum of digit-plane products; each digit-plane
+// operands is the weighted sum of digit-plane products; every digit-plane
 // product is an exact int32 dot product on v_mfma_i32_16x16x64_i8, so the
-// accumulation itself adds no rounding error at all. Digit pairs (k, l) of
-// equal weight k+l share one int32 accumulator.
+// accumulation adds no rounding at all. Digit pairs of equal weight (k+l)
+// share one int32 accumulator ("class").
 //
-// The half-transform keeps the pairs with k + l <= 3 of four digits each
-// (relative truncation about 2^-28); the fifth digit plane of B is kept for
-// the FP64 gamma partials (gamma_Q = sum_mn B D needs more than 28 bits).
+//   half-transform  X = B C^T     4 digits each, pairs k+l <= 3 (10 pairs)
+//   exchange        K = sum_Q X_Q X_Q^T   4 digits, pairs k+l <= 4 with
+//                   k,l <= 3 (13 pairs); the k+l = 4 terms keep the
+//                   same-number cross terms on the diagonal, which do not
+//                   average out
+//   Coulomb gamma   gamma_Q = sum_mn B_Qmn D_mn from 5 digits of B (35 bits)
+// Class sums are combined in FP64 and scaled by the row exponents.
 #include "ozaki.hpp"
 
 namespace nopt {
@@ -26,7 +30,7 @@ __device__ __forceinline__ v4i_t mfma_i8(const uint4& a, const uint4& b, v4i_t c
                                                0, 0, 0);
 }
 
-// row maximum -> exponent e with |x| * 2^-e < 1 (0 for an all-zero row)
+// row maximum -> exponent e with |x| 2^-e < 1 (0 for an all-zero row)
 __device__ __forceinline__ int row_exponent(double mx) {
   if (!(mx > 0.0)) return 0;
   int e;
@@ -34,9 +38,9 @@ __device__ __forceinline__ int row_exponent(double mx) {
   return e;
 }
 
-// Splits the rows of a row-major FP64 matrix (rows x cols, ld) into ns int8
-// digit planes planes[k][row][ldo] (zero padded to ldo) and row exponents.
-// One wavefront per row.
+// Splits the rows of a row-major FP64 matrix (rows_src x cols, ld) into ns
+// int8 digit planes planes[k][row][ldo] (rows_dst rows, zero padded) and row
+// exponents. One wavefront per row.
 __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ src, long ld, int cols,
                                                    long rows_src, int ns, int8_t* __restrict__ planes,
                                                    long ldo, long rows_dst, int* __restrict__ expo) {
@@ -52,7 +56,7 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
   if (lane == 0) expo[row] = e;
   const long plane = rows_dst * ldo;
   for (int c = lane; c < ldo; c += 64) {
-    double r = (live && c < cols) ? ldexp(s[c], 6 - e) : 0.0;  // v * 2^6, |r| <= 64
+    double r = (live && c < cols) ? ldexp(s[c], 6 - e) : 0.0;  // |r| < 64
     for (int k = 0; k < ns; ++k) {
       const double d = rint(r);
       planes[k * plane + row * ldo + c] = (int8_t)d;
@@ -61,59 +65,40 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
   }
 }
 
-// X[Q][m][col] = 2^(e_Qm + f_col) sum_{k+l<=3} 2^(-12-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
-// gpart[Q][mtile] = sum over the tile rows of sum_n B[Q][m][n] D[m][n] with B
-// rebuilt exactly from its five digit planes.
-// Tile: 64 rows (16 per wave) x 16*NT columns, k-steps of 64.
+// ------------------------------------------------------------ half-transform
+// X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=3} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
+// in FP64 (the combination of the four exact class sums is exact to FP64
+// rounding). Tile: 64 rows (16 per wave) x 16*NT columns, k-steps of 64.
 template <int NT>
 __global__ __launch_bounds__(256) void k_half_i8(const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be,
                                                  long ld8, int nbf, const int8_t* __restrict__ Cd, long cplane,
-                                                 const int* __restrict__ Ce, const double* __restrict__ Dp, long ldd,
-                                                 float* __restrict__ X, long ldx, double* __restrict__ gpart,
-                                                 int mtiles) {
+                                                 const int* __restrict__ Ce, double* __restrict__ X, long ldx) {
   constexpr int BMI = 64, BN = 16 * NT, KS = 64, PAD = 16;
   __shared__ __attribute__((aligned(16))) int8_t As[4][BMI][KS + PAD];
   __shared__ __attribute__((aligned(16))) int8_t Cs[4][BN][KS + PAD];
-  __shared__ double red[4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int mtile = blockIdx.x;
   const long Q = blockIdx.y;
   const int cg = blockIdx.z;
   const int m0 = mtile * BMI;
-  // loader role: row ar of the tile, 16-wide k segment aq
-  const int ar = t >> 2, aq = (t & 3) * 16;
+  const int ar = t >> 2, ap = t & 3;  // loader: row ar, digit plane ap (64 bytes)
   const bool arow_ok = m0 + ar < nbf;
-  const long arow = Q * nbf + m0 + ar;
-  const int8_t* Ap = Bd + arow * ld8 + aq;
-  const double* Drow = Dp + (long)(m0 + ar) * ldd + aq;
+  const int8_t* Ap = Bd + ap * bplane + (Q * nbf + m0 + ar) * ld8;
   v4i_t acc[4][NT];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[c][j] = v4i_t{0, 0, 0, 0};
-  double rd_hi = 0.0, rd_lo = 0.0;
-  uint4 a_reg[5];
-  double d_reg[16];
+  uint4 a_reg[4];
   constexpr int CCH = BN * 4 * (KS / 16);  // 16-byte chunks of the C tile
-  uint4 c_reg[(CCH + 255) / 256];
+  constexpr int CPT = (CCH + 255) / 256;
+  uint4 c_reg[CPT];
   auto gload = [&](int k0) {
-    if (arow_ok) {
 #pragma unroll
-      for (int p = 0; p < 5; ++p) a_reg[p] = *reinterpret_cast<const uint4*>(Ap + p * bplane + k0);
+    for (int i = 0; i < 4; ++i)
+      a_reg[i] = arow_ok ? *reinterpret_cast<const uint4*>(Ap + k0 + 16 * i) : uint4{0, 0, 0, 0};
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const double2 x = *reinterpret_cast<const double2*>(Drow + k0 + 2 * i);
-        d_reg[2 * i] = x.x;
-        d_reg[2 * i + 1] = x.y;
-      }
-    } else {
-#pragma unroll
-      for (int p = 0; p < 5; ++p) a_reg[p] = uint4{0, 0, 0, 0};
-#pragma unroll
-      for (int i = 0; i < 16; ++i) d_reg[i] = 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < (CCH + 255) / 256; ++u) {
+    for (int u = 0; u < CPT; ++u) {
       const int e = t + 256 * u;
       if (e < CCH) {
         const int col = e / 16, p = (e / 4) % 4, seg = (e % 4) * 16;
@@ -125,30 +110,13 @@ __global__ __launch_bounds__(256) void k_half_i8(const int8_t* __restrict__ Bd, 
   for (int k0 = 0; k0 < ld8; k0 += KS) {
     __syncthreads();
 #pragma unroll
-    for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(&As[p][ar][aq]) = a_reg[p];
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<uint4*>(&As[ap][ar][16 * i]) = a_reg[i];
 #pragma unroll
-    for (int u = 0; u < (CCH + 255) / 256; ++u) {
+    for (int u = 0; u < CPT; ++u) {
       const int e = t + 256 * u;
       if (e < CCH) {
         const int col = e / 16, p = (e / 4) % 4, seg = (e % 4) * 16;
         *reinterpret_cast<uint4*>(&Cs[p][col][seg]) = c_reg[u];
-      }
-    }
-    // FP64 gamma partials from the exact five-digit value of B:
-    // v 2^6 = d0 + 2^-7 d1 + ... = 2^-7 hi + 2^-28 lo with
-    // hi = 128 d0 + d1 and lo = 2^14 d2 + 2^7 d3 + d4 (both exact integers)
-    {
-      const int8_t* b0 = reinterpret_cast<const int8_t*>(&a_reg[0]);
-      const int8_t* b1 = reinterpret_cast<const int8_t*>(&a_reg[1]);
-      const int8_t* b2 = reinterpret_cast<const int8_t*>(&a_reg[2]);
-      const int8_t* b3 = reinterpret_cast<const int8_t*>(&a_reg[3]);
-      const int8_t* b4 = reinterpret_cast<const int8_t*>(&a_reg[4]);
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int hi = 128 * (int)b0[i] + (int)b1[i];
-        const int lo = 16384 * (int)b2[i] + 128 * (int)b3[i] + (int)b4[i];
-        rd_hi += (double)hi * d_reg[i];
-        rd_lo += (double)lo * d_reg[i];
       }
     }
     __syncthreads();
@@ -175,27 +143,197 @@ __global__ __launch_bounds__(256) void k_half_i8(const int8_t* __restrict__ Bd, 
       acc[3][j] = mfma_i8(av[3], bv[0], acc[3][j]);
     }
   }
-  // epilogue: X in FP64 from the class sums, stored as FP32
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int m = m0 + w * 16 + 4 * (lane / 16) + v;
     if (m < nbf) {
       const int em = Be[Q * nbf + m];
-      float* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
+      double* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         const int col = cg * BN + j * 16 + lane % 16;
         const double s = (double)acc[0][j][v] + 0x1p-7 * (double)acc[1][j][v] + 0x1p-14 * (double)acc[2][j][v] +
                          0x1p-21 * (double)acc[3][j][v];
-        xr[j * 16] = (float)ldexp(s, em + Ce[col] - 12);
+        xr[j * 16] = ldexp(s, em + Ce[col] - 12);
       }
     }
   }
-  if (cg == 0) {
-    const double part = arow_ok ? ldexp(rd_hi, Be[arow] - 13) + ldexp(rd_lo, Be[arow] - 34) : 0.0;
-    const double s = block_sum256(part, red);
-    if (t == 0) gpart[Q * mtiles + mtile] = s;
+}
+
+// ------------------------------------------------------------- Coulomb gamma
+// gamma[Q] = sum_mn B[Q][m][n] D[m][n] over the lower triangle (off-diagonal
+// weight 2) with B rebuilt exactly from its five digit planes:
+//   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
+// One block per auxiliary function; each wave walks row pairs (p, nbf-1-p)
+// in 8-column chunks.
+__global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd, long bplane,
+                                                  const int* __restrict__ Be, long ld8, int nbf,
+                                                  const double* __restrict__ Dp, long ldd,
+                                                  double* __restrict__ gamma) {
+  __shared__ double red[4];
+  const long Q = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc = 0.0;
+  const int npairs = (nbf + 1) / 2;
+  for (int p = w; p < npairs; p += 4) {
+    const int m1 = p, m2 = nbf - 1 - p;
+    const int c1 = m1 / 8 + 1, c2 = (m2 != m1) ? m2 / 8 + 1 : 0;
+    for (int ch = lane; ch < c1 + c2; ch += 64) {
+      const int m = ch < c1 ? m1 : m2;
+      const int n0 = 8 * (ch < c1 ? ch : ch - c1);
+      const long row = Q * nbf + m;
+      const int8_t* b = Bd + row * ld8 + n0;
+      int8_t d[5][8];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(d[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
+      const double* dr = Dp + (long)m * ldd + n0;
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = n0 + j;
+        if (n > m) break;
+        const int hi = 128 * (int)d[0][j] + (int)d[1][j];
+        const int lo = 16384 * (int)d[2][j] + 128 * (int)d[3][j] + (int)d[4][j];
+        const double bv = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
+        s += (n < m ? 2.0 : 1.0) * bv * dr[j];
+      }
+      acc += ldexp(s, Be[row] - 6);
+    }
   }
+  const double tot = block_sum256(acc, red);
+  if (threadIdx.x == 0) gamma[Q] = tot;
+}
+
+// ------------------------------------------------------------------ exchange
+__device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
+  bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  while (bi * (bi + 1) / 2 > t) --bi;
+  bj = t - bi * (bi + 1) / 2;
+}
+
+// Kpart[s][tile][r][c] = sum_{Q in split s} 2^(e_Qm + e_Qn - 12) sum_{k,l}
+//   2^(-7(k+l)) <Xd_k[Q][m][:], Xd_l[Q][n][:]>,  m = 64 bi + r, n = 64 bj + c.
+// Tile 64x64: 4 waves of 32x32 (2x2 MFMA blocks), k-steps of 64 over the
+// padded orbital index.
+constexpr int KT8 = 64;
+__global__ __launch_bounds__(256) void k_syrk_i8(const int8_t* __restrict__ Xd, long xplane,
+                                                 const int* __restrict__ Xe, long ldk, int nbf, int naux,
+                                                 double* __restrict__ Kpart) {
+  constexpr int KS = 64, PAD = 16;
+  __shared__ __attribute__((aligned(16))) int8_t As[4][KT8][KS + PAD];
+  __shared__ __attribute__((aligned(16))) int8_t Bs[4][KT8][KS + PAD];
+  __shared__ int ea[KT8], eb[KT8];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  int bi, bj;
+  tri_tile(blockIdx.x, bi, bj);
+  const int s = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
+  // loader role: row lr of both blocks, digit plane lp (64 bytes per k-step)
+  const int lr = t >> 2, lp = t & 3;
+  const bool ra = bi * KT8 + lr < nbf, rb = bj * KT8 + lr < nbf;
+  v4i_t acc[5][2][2];
+  d4_t k64[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) k64[i][j] = d4_t{0, 0, 0, 0};
+  uint4 a_reg[4], b_reg[4];
+  auto gload = [&](int q, int k0) {
+    const int8_t* pa = Xd + lp * xplane + ((long)q * nbf + bi * KT8 + lr) * ldk + k0;
+    const int8_t* pb = Xd + lp * xplane + ((long)q * nbf + bj * KT8 + lr) * ldk + k0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a_reg[i] = ra ? *reinterpret_cast<const uint4*>(pa + 16 * i) : uint4{0, 0, 0, 0};
+      b_reg[i] = rb ? *reinterpret_cast<const uint4*>(pb + 16 * i) : uint4{0, 0, 0, 0};
+    }
+  };
+  if (q0 < q1) gload(q0, 0);
+  for (int q = q0; q < q1; ++q) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[c][i][j] = v4i_t{0, 0, 0, 0};
+    for (int k0 = 0; k0 < ldk; k0 += KS) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        *reinterpret_cast<uint4*>(&As[lp][lr][16 * i]) = a_reg[i];
+        *reinterpret_cast<uint4*>(&Bs[lp][lr][16 * i]) = b_reg[i];
+      }
+      if (k0 == 0 && t < KT8) {
+        ea[t] = bi * KT8 + t < nbf ? Xe[(long)q * nbf + bi * KT8 + t] : 0;
+        eb[t] = bj * KT8 + t < nbf ? Xe[(long)q * nbf + bj * KT8 + t] : 0;
+      }
+      __syncthreads();
+      {
+        int nq = q, nk = k0 + KS;
+        if (nk >= ldk) {
+          nk = 0;
+          ++nq;
+        }
+        if (nq < q1) gload(nq, nk);
+      }
+      uint4 av[4][2];
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          av[p][i] = *reinterpret_cast<const uint4*>(&As[p][wr * 32 + i * 16 + lane % 16][16 * (lane / 16)]);
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        uint4 bv[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+          bv[p] = *reinterpret_cast<const uint4*>(&Bs[p][wc * 32 + j * 16 + lane % 16][16 * (lane / 16)]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          acc[0][i][j] = mfma_i8(av[0][i], bv[0], acc[0][i][j]);
+          acc[1][i][j] = mfma_i8(av[0][i], bv[1], acc[1][i][j]);
+          acc[1][i][j] = mfma_i8(av[1][i], bv[0], acc[1][i][j]);
+          acc[2][i][j] = mfma_i8(av[0][i], bv[2], acc[2][i][j]);
+          acc[2][i][j] = mfma_i8(av[1][i], bv[1], acc[2][i][j]);
+          acc[2][i][j] = mfma_i8(av[2][i], bv[0], acc[2][i][j]);
+          acc[3][i][j] = mfma_i8(av[0][i], bv[3], acc[3][i][j]);
+          acc[3][i][j] = mfma_i8(av[1][i], bv[2], acc[3][i][j]);
+          acc[3][i][j] = mfma_i8(av[2][i], bv[1], acc[3][i][j]);
+          acc[3][i][j] = mfma_i8(av[3][i], bv[0], acc[3][i][j]);
+          acc[4][i][j] = mfma_i8(av[1][i], bv[3], acc[4][i][j]);
+          acc[4][i][j] = mfma_i8(av[2][i], bv[2], acc[4][i][j]);
+          acc[4][i][j] = mfma_i8(av[3][i], bv[1], acc[4][i][j]);
+        }
+      }
+    }
+    // combine the class sums of this auxiliary function in FP64
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int cc = wc * 32 + j * 16 + lane % 16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rr = wr * 32 + i * 16 + 4 * (lane / 16) + v;
+          const double sv = (double)acc[0][i][j][v] + 0x1p-7 * (double)acc[1][i][j][v] +
+                            0x1p-14 * (double)acc[2][i][j][v] + 0x1p-21 * (double)acc[3][i][j][v] +
+                            0x1p-28 * (double)acc[4][i][j][v];
+          k64[i][j][v] += ldexp(sv, ea[rr] + eb[cc] - 12);
+        }
+      }
+  }
+  double* Kt = Kpart + ((long)s * gridDim.x + blockIdx.x) * KT8 * KT8;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rr = wr * 32 + i * 16 + 4 * (lane / 16) + v;
+        const int cc = wc * 32 + j * 16 + lane % 16;
+        Kt[rr * KT8 + cc] = k64[i][j][v];
+      }
 }
 
 }  // namespace
@@ -208,12 +346,10 @@ hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, in
 }
 
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                          int nbf, const int8_t* Cd, long cplane, const int* Ce, const double* Dp, long ldd,
-                          float* X, long ldx, double* gpart, int mtiles) {
-#define NOPT_HI8(NTV)                                                                                     \
-  case NTV:                                                                                               \
-    k_half_i8<NTV><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, Dp, ldd, X, ldx, gpart, \
-                                        mtiles);                                                          \
+                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx) {
+#define NOPT_HI8(NTV)                                                                                        \
+  case NTV:                                                                                                  \
+    k_half_i8<NTV><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx);                 \
     break;
   switch (nt) {
     NOPT_HI8(1)
@@ -225,6 +361,18 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
       return hipErrorInvalidValue;
   }
 #undef NOPT_HI8
+  return hipGetLastError();
+}
+
+hipError_t launch_gamma_i8(int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8, int nbf,
+                           const double* Dp, long ldd, double* gamma) {
+  k_gamma_i8<<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, gamma);
+  return hipGetLastError();
+}
+
+hipError_t launch_syrk_i8(dim3 grid, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
+                          int naux, double* Kpart) {
+  k_syrk_i8<<<grid, 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, Kpart);
   return hipGetLastError();
 }
 

@@ -17,8 +17,9 @@ def declared_symbols():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"\b(nopt_\w+|DM_to_F_transform_RI|gen_RI_AA\w*|regen_RI_AA|RI_core_realloc|"
-                             r"calc_2el_MO_INTS_RI)\s*\(", text):
+        text = re.sub(r"//[^\n]*", "", text)
+        # every function declaration: a line that opens with its return type
+        for m in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b(\w+)\s*\(", text, flags=re.M):
             names.add(m.group(1))
     return sorted(names)
 

@@ -132,3 +132,38 @@ def test_energy_gradient_matches_finite_differences():
             fd[i, mu] = (ctx.energy_gradient(H, Cp)[0] - ctx.energy_gradient(H, Cm)[0]) / (2 * h)
             say(i, mu, Gr[i, mu], fd[i, mu])
             assert abs(Gr[i, mu] - fd[i, mu]) < 1e-7 * max(1.0, abs(fd[i, mu]))
+
+
+def test_nopt_ri_entry_points_match_oracle(c2h4_case):
+    """NOPT's own call sequence through include/nopt_ri.h: hand over (P|Q) and
+    RI_AA in NOPT's layout, then DM_to_F_transform_RI and the J/K half of
+    calc_2el_MO_INTS_RI, against the oracle's restatement of src/RI.cpp:115-176."""
+    from nopt_amd import _lib as L
+    om, r, pm = c2h4_case
+    lib = L.lib()
+    D, Cc = r["D"], r["C"]
+    MO = np.ascontiguousarray(Cc.T)
+    J0, K0 = r["jk"](D, MO, om.nocc)
+    V, A = pm.metric(), np.ascontiguousarray(pm.eri3_nopt())
+    n = pm.nbf
+    assert lib.RI_core_realloc(om.nocc, n) < 0          # no context yet
+    assert lib.RI_gpu_attach(L.dptr(V), L.dptr(A), n, pm.naux, 0) == 0
+    try:
+        assert lib.RI_core_realloc(om.nocc, n) == 0
+        J, K = np.zeros((n, n)), np.zeros((n, n))
+        assert lib.DM_to_F_transform_RI(L.dptr(J), L.dptr(K), L.dptr(np.ascontiguousarray(D)), L.dptr(MO),
+                                        om.nocc, n) == 1
+        say("RI shim max|dJ|", np.abs(J - J0).max(), "max|dK|", np.abs(K - K0).max())
+        assert np.abs(J - J0).max() < 1e-10 and np.abs(K - K0).max() < 1e-10
+        J2, K2 = np.full((n, n), 7.0), np.full((n, n), 7.0)
+        rc = lib.calc_2el_MO_INTS_RI(n, L.dptr(np.ascontiguousarray(D)), L.dptr(J2), L.dptr(K2), None,
+                                     L.dptr(MO), None, None, om.nocc, 0, 0)
+        assert rc == 0
+        assert np.abs(J2 - J).max() == 0.0 and np.abs(K2 - K).max() == 0.0
+        assert lib.calc_2el_MO_INTS_RI(n, None, None, None, None, None, None, None, om.nocc, 0, 0) == 0
+        assert lib.DM_to_F_transform_RI(L.dptr(J), L.dptr(K), L.dptr(np.ascontiguousarray(D)), L.dptr(MO),
+                                        om.nocc, n + 1) < 0
+    finally:
+        lib.regen_RI_AA_release()
+    assert lib.DM_to_F_transform_RI(L.dptr(J), L.dptr(K), L.dptr(np.ascontiguousarray(D)), L.dptr(MO),
+                                    om.nocc, n) < 0
