@@ -45,7 +45,8 @@ extern "C" {
 typedef struct nopt_jk_ctx nopt_jk_ctx;
 
 /* Creates a context on HIP device `device` for nbf orbital functions and naux
- * auxiliary functions. Nothing is allocated on the device until set_B.
+ * auxiliary functions, with its own stream and the small per-build buffers;
+ * B and the per-build tensors are allocated by set_B and the first build.
  * Replaces the allocation part of gen_RI_AA (src/RI.cpp:34-79). */
 int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out);
 void nopt_jk_destroy(nopt_jk_ctx* ctx);
@@ -83,6 +84,23 @@ int nopt_jk_build(nopt_jk_ctx* ctx, const double* D, const double* C, long nocc,
 int nopt_jk_build_device(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc,
                          double* dJ, double* dK, int prec, void* stream);
 
+/* Several builds against the same B in one pass (NOPT calls the build twice
+ * per CASSCF macro-iteration, for the inactive and the total density,
+ * src/CAS.cpp:696-707, and a signed factor of an indefinite density needs a
+ * positive and a negative orbital factor). Density d uses D[d], the first
+ * nocc[d] rows of C[d], and writes J[d] and K[d]; the array pointers J, K, D,
+ * C may be NULL and so may individual entries, with the rules of
+ * nopt_jk_build per entry. The orbital factors are stacked so that the
+ * half-transform reads B once, and gamma and J are formed for all densities
+ * in one pass over B each. 1 <= nd <= NOPT_MAX_BATCH. */
+#define NOPT_MAX_BATCH 4
+int nopt_jk_build_batch(nopt_jk_ctx* ctx, int nd, const double* const* D, const double* const* C,
+                        const long* nocc, double* const* J, double* const* K, int prec);
+/* The batched build on device buffers, enqueued on `stream`. */
+int nopt_jk_build_batch_device(nopt_jk_ctx* ctx, int nd, const double* const* dD,
+                               const double* const* dC, const long* nocc, double* const* dJ,
+                               double* const* dK, int prec, void* stream);
+
 /* Fock assembly and energy on the device from host buffers:
  * F = H + cj*J - ck*K and E = sum D.*F + sum D.*H (E_1el_calc,
  * src/doCI_matr.cpp:38-47, as used at src/SCF.cpp:134; the caller adds
@@ -100,9 +118,10 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, 
 
 /* Per-phase timing with HIP events on the build's stream (no counterpart in
  * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When
- * enabled, each build records events around its five phases: copy-in,
- * half-transform (with the fused gamma partials), Coulomb, exchange and
- * finalize. read_timing returns the summed milliseconds per phase (ms[5])
+ * enabled, each build (or batch) records events around its five phases:
+ * staging of D and C, half-transform (with the fused gamma partials of a
+ * single FP64/FP32/bf16 build), Coulomb (gamma and J partials), exchange (K
+ * products and K output) and the J output. read_timing returns the summed milliseconds per phase (ms[5])
  * and the number of builds recorded since the last read, then resets. */
 int nopt_jk_set_timing(nopt_jk_ctx* ctx, int enable);
 int nopt_jk_read_timing(nopt_jk_ctx* ctx, double* ms, long* builds);

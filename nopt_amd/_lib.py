@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libnoptjk.so")
 BASIS_DIR = os.path.join(HERE, "basis")
 
 NOPT_OK, NOPT_EINVAL, NOPT_ENOMEM, NOPT_ECHOL, NOPT_EHIP, NOPT_EIO, NOPT_ESTATE = range(7)
+MAX_BATCH = 4  # NOPT_MAX_BATCH
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3, "int8": 4}
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -51,6 +52,12 @@ GPU_SYMBOLS = [
     ("nopt_jk_build", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, _c_double_p,
                                      _c_double_p, ctypes.c_int]),
     ("nopt_jk_build_device", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp, _vp, ctypes.c_int, _vp]),
+    ("nopt_jk_build_batch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_c_double_p),
+                                           ctypes.POINTER(_c_double_p), _c_long_p, ctypes.POINTER(_c_double_p),
+                                           ctypes.POINTER(_c_double_p), ctypes.c_int]),
+    ("nopt_jk_build_batch_device", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  _c_long_p, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  ctypes.c_int, _vp]),
     ("nopt_jk_fock_energy", ctypes.c_int, [_vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p,
                                            ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     ("nopt_jk_energy_gradient", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_int,

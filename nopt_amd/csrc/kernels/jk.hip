@@ -239,6 +239,47 @@ __global__ __launch_bounds__(256) void k_half(const double* __restrict__ B, long
 }
 
 // ------------------------------------------------------------------ gamma
+// gamma[d][Q] = sum_mn B[Q][m][n] D_d[m][n] for ND densities (D_d at
+// Dp + d*dstride) over the lower triangle of B (off-diagonal weight 2): one
+// block per auxiliary function, row pairs (p, nbf-1-p) in 8-column chunks, so
+// B is read once for all ND densities.
+template <int ND>
+__global__ __launch_bounds__(256) void k_gamma_f64(const double* __restrict__ B, long ld, int nbf,
+                                                   const double* __restrict__ Dp, long ldd, long dstride,
+                                                   double* __restrict__ gamma, long gstride) {
+  __shared__ double red[4];
+  const long Q = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double acc[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+  const int npairs = (nbf + 1) / 2;
+  for (int p = w; p < npairs; p += 4) {
+    const int m1 = p, m2 = nbf - 1 - p;
+    const int c1 = m1 / 8 + 1, c2 = (m2 != m1) ? m2 / 8 + 1 : 0;
+    for (int ch = lane; ch < c1 + c2; ch += 64) {
+      const int m = ch < c1 ? m1 : m2;
+      const int n0 = 8 * (ch < c1 ? ch : ch - c1);
+      double bv[8];
+      load8<double>(B + ((long)Q * nbf + m) * ld + n0, bv);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bv[j] *= (n0 + j < m) ? 2.0 : (n0 + j == m ? 1.0 : 0.0);
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double dr[8];
+        load8<double>(Dp + d * dstride + (long)m * ldd + n0, dr);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[d] += bv[j] * dr[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double tot = block_sum256(acc[d], red);
+    if (threadIdx.x == 0) gamma[d * gstride + Q] = tot;
+  }
+}
+
 __global__ void k_gamma(const double* __restrict__ gpart, int mtiles, int naux,
                         double* __restrict__ gamma) {
   const int Q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -249,49 +290,65 @@ __global__ void k_gamma(const double* __restrict__ gpart, int mtiles, int naux,
 }
 
 // ---------------------------------------------------------------- Coulomb
-// Jpart[s][m][n] = sum_{Q in split s} gamma_Q B[Q][m][n] over the lower
-// triangle. A block owns the row pair (p, nbf-1-p): nbf+1 elements, read as
-// aligned pairs of columns (a row's last pair may include its first upper
-// element, which lies inside the row because ld > nbf and is not written).
-// The Q loop is unrolled so that UNR loads per thread are in flight.
+// Jpart[d][s][m][n] = sum_{Q in split s} gamma[d][Q] B[Q][m][n] over the lower
+// triangle, for ND densities at once (B is read once). A block owns the row
+// pair (p, nbf-1-p): nbf+1 elements, read as aligned pairs of columns (a
+// row's last pair may include its first upper element, which lies inside the
+// row because ld > nbf and is not written). The Q loop is unrolled so that UNR
+// loads per thread are in flight.
+template <int ND>
 __global__ __launch_bounds__(256) void k_jpart(const double* __restrict__ B, long ld, int nbf, int naux,
-                                               const double* __restrict__ gamma,
-                                               double* __restrict__ Jpart) {
-  constexpr int UNR = 8;
-  __shared__ double g[JCHUNK_MAX];
+                                               const double* __restrict__ gamma, long gstride,
+                                               double* __restrict__ Jpart, long jstride) {
+  constexpr int UNR = ND == 1 ? 8 : 4;
+  __shared__ double gs[JCHUNK_MAX];  // the host keeps ND * (q1 - q0) <= JCHUNK_MAX
   const int p = blockIdx.x, s = blockIdx.y, ns = gridDim.y;
   const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
-  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x) g[q - q0] = gamma[q];
+  const int len = q1 - q0;
+  for (int q = q0 + threadIdx.x; q < q1; q += blockDim.x)
+#pragma unroll
+    for (int d = 0; d < ND; ++d) gs[d * len + q - q0] = gamma[d * gstride + q];
   __syncthreads();
+  auto g = [&](int d, int i) { return gs[d * len + i]; };
   const int m1 = p, m2 = nbf - 1 - p;
   const int s1 = (m1 + 2) / 2, s2 = (m2 != m1) ? (m2 + 2) / 2 : 0;
   const long qstride = (long)nbf * ld;
-  double* Js = Jpart + (long)s * nbf * nbf;
   for (int slot = threadIdx.x; slot < s1 + s2; slot += blockDim.x) {
     const int row = slot < s1 ? m1 : m2;
     const int col = 2 * (slot < s1 ? slot : slot - s1);
     const double* bp = B + (long)q0 * qstride + (long)row * ld + col;
-    double a0 = 0.0, a1 = 0.0;
+    double a0[ND], a1[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) a0[d] = a1[d] = 0.0;
     int q = q0;
     for (; q + UNR <= q1; q += UNR) {
       double2 v[UNR];
 #pragma unroll
       for (int u = 0; u < UNR; ++u) v[u] = *reinterpret_cast<const double2*>(bp + (long)u * qstride);
 #pragma unroll
-      for (int u = 0; u < UNR; ++u) {
-        a0 += g[q - q0 + u] * v[u].x;
-        a1 += g[q - q0 + u] * v[u].y;
-      }
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          a0[d] += g(d, q - q0 + u) * v[u].x;
+          a1[d] += g(d, q - q0 + u) * v[u].y;
+        }
       bp += (long)UNR * qstride;
     }
     for (; q < q1; ++q) {
       const double2 v = *reinterpret_cast<const double2*>(bp);
-      a0 += g[q - q0] * v.x;
-      a1 += g[q - q0] * v.y;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        a0[d] += g(d, q - q0) * v.x;
+        a1[d] += g(d, q - q0) * v.y;
+      }
       bp += qstride;
     }
-    Js[(long)row * nbf + col] = a0;
-    if (col + 1 <= row) Js[(long)row * nbf + col + 1] = a1;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double* Js = Jpart + d * jstride + (long)s * nbf * nbf;
+      Js[(long)row * nbf + col] = a0[d];
+      if (col + 1 <= row) Js[(long)row * nbf + col + 1] = a1[d];
+    }
   }
 }
 
@@ -473,6 +530,10 @@ __global__ void k_density(const double* __restrict__ C, int nocc, int nbf, doubl
 }  // namespace
 
 // ================================================================ context
+namespace {
+constexpr int kMaxBatch = NOPT_MAX_BATCH;  // densities per batched build
+}  // namespace
+
 struct nopt_jk_ctx {
   int device = 0;
   long nbf = 0, naux = 0, npair = 0;
@@ -484,14 +545,22 @@ struct nopt_jk_ctx {
   long cap_ldx = 0;
   void* X = nullptr;  // naux*nbf*ldx elements of the largest type
   size_t X_bytes = 0;
-  double *gpart = nullptr, *gamma = nullptr, *Jpart = nullptr, *Kpart = nullptr;
-  int nsk = 0, nsj = 0;
-  double *Dp = nullptr;  // nbf x ld64 padded density
+  double *gpart = nullptr, *gamma = nullptr, *Kpart = nullptr;
+  void* Jpart = nullptr;  // J partials: (densities x splits) x nbf x nbf
+  size_t Jpart_bytes = 0;
+  int nsk = 0;
+  double* Dp = nullptr;  // padded densities, kMaxBatch slots of dstride
+  long dstride = 0;
   void* Cp = nullptr;    // padded orbital factor
   size_t Cp_bytes = 0;
+  // host-staging slots of the host-memory entry points (slot 0 also serves
+  // the Fock/energy helpers)
   double *dJ = nullptr, *dK = nullptr, *dD = nullptr, *dC = nullptr, *dH = nullptr,
          *dF = nullptr, *dG = nullptr, *epart = nullptr, *dE = nullptr;
   long dC_cap = 0;
+  double *xD[kMaxBatch - 1] = {}, *xC[kMaxBatch - 1] = {}, *xJ[kMaxBatch - 1] = {}, *xK[kMaxBatch - 1] = {};
+  long xD_cap[kMaxBatch - 1] = {}, xC_cap[kMaxBatch - 1] = {}, xJ_cap[kMaxBatch - 1] = {},
+       xK_cap[kMaxBatch - 1] = {};
   int* info = nullptr;
   // int8 digit planes of B (5 x naux*nbf x ld8) with per-row exponents, and
   // the per-build digit planes of the orbital factor (4 x ldx x ld8)
@@ -611,135 +680,221 @@ int finish_B(nopt_jk_ctx* c, const double* dBp) {
     if (!rc) rc = dev_alloc((void**)&c->Be, sizeof(int) * rows, "B row exponents");
     if (rc) return rc;
   }
-  HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)c->nbf, rows, 5, c->Bd, c->ld8, rows, c->Be, c->stream));
+  HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)c->nbf, rows, 5, c->Bd, c->ld8, rows, rows * c->ld8, c->Be,
+                             c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return NOPT_OK;
 }
 
-// Enqueues one build on device buffers. dD: nbf x nbf, dC: nocc x nbf.
-int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, double* dJ,
-                  double* dK, int prec, hipStream_t s) {
+// One density of a batched build (device pointers). D: nbf x nbf; C: nocc x
+// nbf (orbitals as rows); J, K: nbf x nbf outputs, either may be null.
+struct BuildItem {
+  const double* D;
+  const double* C;
+  long nocc;
+  double* J;
+  double* K;
+};
+
+template <typename T>
+void stage_rows(const double* src, long lds, int rows, int cols, T* dst, long ldd, int rows_pad, hipStream_t s) {
+  const long n = (long)rows_pad * ldd;
+  if (n > 0) k_pad_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, lds, rows, cols, dst, ldd, rows_pad);
+}
+
+// Enqueues the J/K builds of nd densities that share B. The orbital factors
+// are stacked into one padded factor, so the half-transform reads B (or its
+// digit planes) once for all of them; gamma and J are formed for every
+// density in one pass over B each; K is one triangular product per density.
+// Phases (for the optional timing): staging, half-transform, Coulomb (gamma +
+// J partials), exchange (K products and K output), J output.
+int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStream_t s) {
   const int nbf = (int)c->nbf, naux = (int)c->naux;
-  const bool wantK = dK != nullptr && nocc > 0;
-  const bool wantJ = dJ != nullptr;
   if (prec < 0 || prec > 4) return set_error(NOPT_EINVAL, "unknown precision mode");
-  if (wantK && !dC) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
-  if (wantJ && !dD) return set_error(NOPT_EINVAL, "J requested without a density");
-  if (!wantJ && !wantK) return NOPT_OK;
+  if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
+  const bool f64 = prec == NOPT_PREC_FP64, i8 = prec == NOPT_PREC_INT8;
+  const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
+  const long unit = bf ? 32 : 16;  // column padding of each orbital block
+  long off[kMaxBatch + 1];
+  int jl[kMaxBatch];
+  int nj = 0;
+  off[0] = 0;
+  for (int d = 0; d < nd; ++d) {
+    const BuildItem& x = it[d];
+    if (x.nocc < 0 || x.nocc > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
+    const bool wk = x.K && x.nocc > 0;
+    if (wk && !x.C) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
+    if (x.J && !x.D) return set_error(NOPT_EINVAL, "J requested without a density");
+    off[d + 1] = off[d] + (wk ? round_up(x.nocc, unit) : 0);
+    if (x.J) jl[nj++] = d;
+  }
+  const long ncol = off[nd];
+  const bool anyK = ncol > 0;
+  // one density with J and K in a non-integer mode: gamma rides on the
+  // half-transform (row dots of the B panels already in registers)
+  const bool fused = !i8 && nd == 1 && nj == 1 && anyK;
   int ngroups = 1;
-  const int nt = nt_for(wantK ? nocc : 1, &ngroups);
-  const long ldx = (long)nt * 16 * ngroups;  // padded orbital count
-  const bool f64 = prec == NOPT_PREC_FP64;
-  const bool i8 = prec == NOPT_PREC_INT8;
-  const long ldb = i8 ? c->ld8 : c->ld64;  // leading dimension of the padded D and C copies
+  const int nt = nt_for(anyK ? ncol : 1, &ngroups);
+  const long ldx = (long)nt * 16 * ngroups;  // stacked, padded orbital count
+  const long ldb = i8 ? c->ld8 : c->ld64;    // leading dimension of the staged D and C
   const int mt = i8 ? c->mtiles64 : c->mtiles;
   const size_t esz = (f64 || i8) ? sizeof(double) : sizeof(float);
+  const long rows = c->naux * c->nbf;
+  const long nn = (long)nbf * nbf;
   int rc;
-  size_t cap = c->X_bytes;
-  rc = ensure(&c->X, &cap, esz * c->naux * c->nbf * ldx, "half-transformed tensor X");
-  if (rc) return rc;
-  c->X_bytes = cap;
-  size_t cpb = c->Cp_bytes;
-  rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
-  if (rc) return rc;
-  c->Cp_bytes = cpb;
-  if (i8) {
-    size_t b1 = c->Cd_bytes, b2 = c->Ce_bytes;
-    rc = ensure(&c->Cd, &b1, (size_t)4 * ldx * c->ld8, "orbital digit planes");
-    if (!rc) rc = ensure(&c->Ce, &b2, sizeof(int) * ldx, "orbital exponents");
+  if (anyK) {
+    size_t cap = c->X_bytes;
+    rc = ensure(&c->X, &cap, esz * rows * ldx, "half-transformed tensor X");
     if (rc) return rc;
-    c->Cd_bytes = b1;
-    c->Ce_bytes = b2;
+    c->X_bytes = cap;
+    size_t cpb = c->Cp_bytes;
+    rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
+    if (rc) return rc;
+    c->Cp_bytes = cpb;
+    if (i8) {
+      size_t b1 = c->Cd_bytes, b2 = c->Ce_bytes;
+      rc = ensure(&c->Cd, &b1, (size_t)4 * ldx * c->ld8, "orbital digit planes");
+      if (!rc) rc = ensure(&c->Ce, &b2, sizeof(int) * ldx, "orbital exponents");
+      if (rc) return rc;
+      c->Cd_bytes = b1;
+      c->Ce_bytes = b2;
+      long ldk_max = 0;
+      for (int d = 0; d < nd; ++d)
+        if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, round_up(it[d].nocc, 64));
+      size_t b3 = c->Xd_bytes, b4 = c->Xe_bytes;
+      rc = ensure(&c->Xd, &b3, (size_t)4 * rows * ldk_max, "exchange digit planes");
+      if (!rc) rc = ensure(&c->Xe, &b4, sizeof(int) * rows, "exchange row exponents");
+      if (rc) return rc;
+      c->Xd_bytes = b3;
+      c->Xe_bytes = b4;
+    }
   }
+  // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
+  const int nsj = nj ? std::max(NSPLIT_J_MIN, (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj)) : 0;
+  if (nj) {
+    size_t jb = c->Jpart_bytes;
+    rc = ensure(&c->Jpart, &jb, sizeof(double) * nj * nsj * nn, "J partials");
+    if (rc) return rc;
+    c->Jpart_bytes = jb;
+  }
+  double* Jpart = (double*)c->Jpart;
   const bool tm = c->timing && c->timed < kTimedBuilds;
   auto mark = [&](int ph) {
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
   };
   mark(0);
-  // padded copies of D (FP64, ld = ldb) and C (compute type)
-  {
-    const long n = (long)nbf * ldb;
-    if (dD)
-      k_pad_copy<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(dD, nbf, nbf, nbf, c->Dp, ldb, nbf);
-    else
-      k_pad_copy<double><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(c->Dp, ldb, 0, 0, c->Dp, ldb, nbf);
-    const long nc = ldx * ldb;
-    const int rows = wantK ? (int)nocc : 0;
+  // densities of the J list into slots 0..nj-1 (ld = ldb); orbital blocks
+  // into the stacked factor at column offsets off[d] (zero rows between)
+  for (int k = 0; k < nj; ++k) stage_rows<double>(it[jl[k]].D, nbf, nbf, nbf, c->Dp + k * c->dstride, ldb, nbf, s);
+  if (anyK) {
+    for (int d = 0; d < nd; ++d) {
+      const int pad = (int)(off[d + 1] - off[d]);
+      if (!pad) continue;
+      const int r = (int)it[d].nocc;
+      if (i8)
+        HIPCHK(nopt::split_rows_i8(it[d].C, nbf, nbf, r, 4, (int8_t*)c->Cd + off[d] * c->ld8, c->ld8, pad,
+                                   ldx * c->ld8, (int*)c->Ce + off[d], s));
+      else if (f64)
+        stage_rows<double>(it[d].C, nbf, r, nbf, (double*)c->Cp + off[d] * ldb, ldb, pad, s);
+      else
+        stage_rows<float>(it[d].C, nbf, r, nbf, (float*)c->Cp + off[d] * ldb, ldb, pad, s);
+    }
+    const int tail = (int)(ldx - ncol);
     if (i8)
-      HIPCHK(nopt::split_rows_i8(dC ? dC : c->Dp, nbf, nbf, rows, 4, (int8_t*)c->Cd, c->ld8, ldx, (int*)c->Ce, s));
+      HIPCHK(nopt::split_rows_i8(c->Dp, ldb, 0, 0, 4, (int8_t*)c->Cd + ncol * c->ld8, c->ld8, tail, ldx * c->ld8,
+                                 (int*)c->Ce + ncol, s));
     else if (f64)
-      k_pad_copy<double><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(dC ? dC : c->Dp, nbf, rows, nbf,
-                                                                      (double*)c->Cp, ldb, (int)ldx);
+      stage_rows<double>(c->Dp, ldb, 0, 0, (double*)c->Cp + ncol * ldb, ldb, tail, s);
     else
-      k_pad_copy<float><<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(dC ? dC : c->Dp, nbf, rows, nbf,
-                                                                     (float*)c->Cp, ldb, (int)ldx);
-    HIPCHK(hipGetLastError());
+      stage_rows<float>(c->Dp, ldb, 0, 0, (float*)c->Cp + ncol * ldb, ldb, tail, s);
   }
-  // half-transform (+ gamma partials)
+  HIPCHK(hipGetLastError());
   mark(1);
-  dim3 gh(mt, naux, wantK ? ngroups : 1);
-  hipError_t e;
-  const long bplane = c->naux * c->nbf * c->ld8;
-  if (i8)
-    e = wantK ? nopt::launch_half_i8(nt, gh, s, c->Bd, bplane, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
-                                     ldx * c->ld8, (const int*)c->Ce, (double*)c->X, ldx)
-              : hipSuccess;
-  else if (prec == NOPT_PREC_FP64)
-    e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X,
-                            ldx, c->gpart, c->mtiles);
-  else if (prec == NOPT_PREC_FP32)
-    e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
-                           c->gpart, c->mtiles);
-  else
-    e = nopt::launch_half_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, nt, gh, s, c->B64, ldb, nbf,
-                               (const float*)c->Cp, c->Dp, (float*)c->X, ldx, c->gpart, c->mtiles);
-  HIPCHK(e);
-  mark(2);
-  if (wantJ) {
+  if (anyK) {
+    dim3 gh(mt, naux, ngroups);
+    hipError_t e;
     if (i8)
-      HIPCHK(nopt::launch_gamma_i8(naux, s, c->Bd, bplane, c->Be, c->ld8, nbf, c->Dp, ldb, c->gamma));
+      e = nopt::launch_half_i8(nt, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
+                               ldx * c->ld8, (const int*)c->Ce, (double*)c->X, ldx);
+    else if (f64)
+      e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X, ldx,
+                              c->gpart, c->mtiles);
+    else if (prec == NOPT_PREC_FP32)
+      e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
+                             c->gpart, c->mtiles);
     else
-      k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, mt, naux, c->gamma);
-    dim3 gj((nbf + 1) / 2, c->nsj);
-    k_jpart<<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, c->Jpart);
+      e = nopt::launch_half_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, nt, gh, s, c->B64, ldb, nbf,
+                                 (const float*)c->Cp, c->Dp, (float*)c->X, ldx, c->gpart, c->mtiles);
+    HIPCHK(e);
+  }
+  mark(2);
+  if (nj) {
+    if (fused) {
+      k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, c->mtiles, naux, c->gamma);
+    } else if (i8) {
+      HIPCHK(nopt::launch_gamma_i8(nj, naux, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, c->Dp, ldb, c->dstride,
+                                   c->gamma, naux));
+    } else {
+#define NOPT_GF(ND)                                                                                        \
+  case ND:                                                                                                 \
+    k_gamma_f64<ND><<<naux, 256, 0, s>>>(c->B64, c->ld64, nbf, c->Dp, ldb, c->dstride, c->gamma, naux); \
+    break;
+      switch (nj) { NOPT_GF(1) NOPT_GF(2) NOPT_GF(3) NOPT_GF(4) }
+#undef NOPT_GF
+    }
+    dim3 gj((nbf + 1) / 2, nsj);
+#define NOPT_JP(ND)                                                                                  \
+  case ND:                                                                                           \
+    k_jpart<ND><<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, naux, Jpart, (long)nsj * nn); \
+    break;
+    switch (nj) { NOPT_JP(1) NOPT_JP(2) NOPT_JP(3) NOPT_JP(4) }
+#undef NOPT_JP
     HIPCHK(hipGetLastError());
   }
   mark(3);
-  if (wantK) {
-    const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
-    const int kpad = (int)round_up(nocc, bf ? 32 : 16);
-    dim3 gk(c->ntiles, c->nsk);
+  for (int d = 0; d < nd; ++d) {
+    if (!it[d].K) continue;
+    const int kpad = (int)(off[d + 1] - off[d]);
+    if (!kpad) {  // no occupied orbitals: K = 0
+      stage_rows<double>(c->Dp, nbf, 0, 0, it[d].K, nbf, nbf, s);
+      continue;
+    }
+    const unsigned nel = (unsigned)((nn + 255) / 256);
     if (i8) {
-      // four digit planes per row (Q, m) of X, then the integer exchange
-      const long ldk = round_up(nocc, 64), rows = c->naux * c->nbf;
-      size_t b1 = c->Xd_bytes, b2 = c->Xe_bytes;
-      rc = ensure(&c->Xd, &b1, (size_t)4 * rows * ldk, "exchange digit planes");
-      if (!rc) rc = ensure(&c->Xe, &b2, sizeof(int) * rows, "exchange row exponents");
-      if (rc) return rc;
-      c->Xd_bytes = b1;
-      c->Xe_bytes = b2;
-      HIPCHK(nopt::split_rows_i8((const double*)c->X, ldx, (int)nocc, rows, 4, (int8_t*)c->Xd, ldk, rows,
-                                 (int*)c->Xe, s));
+      const long ldk = round_up(it[d].nocc, 64);
+      HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)c->Xd, ldk,
+                                 rows, rows * ldk, (int*)c->Xe, s));
       HIPCHK(nopt::launch_syrk_i8(dim3(c->ntiles8, c->nsk8), s, (const int8_t*)c->Xd, rows * ldk,
                                   (const int*)c->Xe, ldk, nbf, naux, c->Kpart));
-    } else if (prec == NOPT_PREC_FP64)
-      k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X, ldx, nbf, naux, kpad, c->Kpart);
-    else if (prec == NOPT_PREC_FP32)
-      k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X, ldx, nbf, naux, kpad, c->Kpart);
-    else
-      HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)c->X, ldx, nbf,
-                                    naux, kpad, c->Kpart));
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
+                                     it[d].K);
+    } else {
+      dim3 gk(c->ntiles, c->nsk);
+      if (f64)
+        k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X + off[d], ldx, nbf, naux, kpad, c->Kpart);
+      else if (prec == NOPT_PREC_FP32)
+        k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X + off[d], ldx, nbf, naux, kpad, c->Kpart);
+      else
+        HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)c->X + off[d], ldx,
+                                      nbf, naux, kpad, c->Kpart));
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk, c->ntiles, KT, nbf, nullptr, it[d].K);
+    }
     HIPCHK(hipGetLastError());
   }
   mark(4);
-  const long nn = (long)nbf * nbf;
-  k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(
-      c->Jpart, c->nsj, c->Kpart, i8 ? c->nsk8 : c->nsk, i8 ? c->ntiles8 : c->ntiles,
-      i8 ? nopt::kSyrkI8Tile : KT, nbf, wantJ ? dJ : nullptr, wantK ? dK : nullptr);
+  for (int k = 0; k < nj; ++k)
+    k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(Jpart + k * nsj * nn, nsj, nullptr, 0, 1, KT, nbf,
+                                                            it[jl[k]].J, nullptr);
   HIPCHK(hipGetLastError());
   mark(5);
   if (tm) ++c->timed;
   return NOPT_OK;
+}
+
+int enqueue_build(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, double* dJ, double* dK, int prec,
+                  hipStream_t s) {
+  const BuildItem x{dD, dC, nocc, dJ, dK};
+  return enqueue_batch(c, 1, &x, prec, s);
 }
 
 int upload(nopt_jk_ctx* c, double** dst, long* cap, const double* src, long count) {
@@ -786,7 +941,6 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
     c->ntiles8 = kt8 * (kt8 + 1) / 2;
     c->nsk8 = std::max(1, std::min((int)naux, (2048 + c->ntiles8 - 1) / c->ntiles8));
   }
-  c->nsj = std::max(NSPLIT_J_MIN, (int)((naux + JCHUNK_MAX - 1) / JCHUNK_MAX));
   int rc = NOPT_OK;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -796,14 +950,14 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   }
   const long nn = nbf * nbf;
   if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles64, "gamma partials");
-  if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * naux, "gamma");
-  if (!rc) rc = dev_alloc((void**)&c->Jpart, sizeof(double) * c->nsj * nn, "J partials");
+  if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * kMaxBatch * naux, "gamma");
   if (!rc)
     rc = dev_alloc((void**)&c->Kpart,
                    sizeof(double) * std::max((long)c->nsk * c->ntiles * KT * KT,
                                              (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile),
                    "K partials");
-  if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * nbf * std::max(c->ld64, c->ld8), "padded density");
+  c->dstride = nbf * std::max(c->ld64, c->ld8);
+  if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * kMaxBatch * c->dstride, "padded densities");
   if (!rc) rc = dev_alloc((void**)&c->dJ, sizeof(double) * nn, "J");
   if (!rc) rc = dev_alloc((void**)&c->dK, sizeof(double) * nn, "K");
   if (!rc) rc = dev_alloc((void**)&c->dD, sizeof(double) * nn, "D");
@@ -848,6 +1002,12 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->Ce);
   dev_free(c->Xd);
   dev_free(c->Xe);
+  for (int i = 0; i < kMaxBatch - 1; ++i) {
+    dev_free(c->xD[i]);
+    dev_free(c->xC[i]);
+    dev_free(c->xJ[i]);
+    dev_free(c->xK[i]);
+  }
   for (auto& ev : c->tev) (void)hipEventDestroy(ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -993,10 +1153,11 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   long b = 0;
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
   if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf;
-  b += (long)c->X_bytes + (long)c->Cp_bytes + (long)c->Cd_bytes + (long)c->Ce_bytes;
-  b += sizeof(double) * (c->naux * c->mtiles64 + c->naux + (long)c->nsj * c->nbf * c->nbf +
-                         (long)c->nsk * c->ntiles * KT * KT + c->nbf * std::max(c->ld64, c->ld8) +
-                         5 * c->nbf * c->nbf);
+  b += (long)(c->X_bytes + c->Cp_bytes + c->Cd_bytes + c->Ce_bytes + c->Xd_bytes + c->Xe_bytes + c->Jpart_bytes);
+  b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux +
+                         std::max((long)c->nsk * c->ntiles * KT * KT,
+                                  (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile) +
+                         kMaxBatch * c->dstride + 5 * c->nbf * c->nbf);
   return b;
 }
 
@@ -1005,37 +1166,89 @@ int nopt_jk_build_device(nopt_jk_ctx* c, const double* dD, const double* dC, lon
   nopt::clear_error();
   int rc = check_ctx(c, true);
   if (rc) return rc;
-  if (nocc < 0 || nocc > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   return enqueue_build(c, dD, dC, nocc, dJ, dK, prec, s);
 }
 
-int nopt_jk_build(nopt_jk_ctx* c, const double* D, const double* C, long nocc, double* J, double* K,
-                  int prec) {
+int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, const double* const* dC,
+                               const long* nocc, double* const* dJ, double* const* dK, int prec, void* stream) {
   nopt::clear_error();
   int rc = check_ctx(c, true);
   if (rc) return rc;
-  if (nocc < 0 || nocc > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
-  const long nn = c->nbf * c->nbf;
-  if (J && !D) return set_error(NOPT_EINVAL, "J requested without a density");
-  if (K && nocc > 0 && !C) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
-  if (D) HIPCHK(hipMemcpyAsync(c->dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-  if (C && nocc > 0) {
-    rc = upload(c, &c->dC, &c->dC_cap, C, nocc * c->nbf);
-    if (rc) return rc;
-  }
-  rc = enqueue_build(c, D ? c->dD : nullptr, (C && nocc > 0) ? c->dC : nullptr, nocc, J ? c->dJ : nullptr,
-                     K ? c->dK : nullptr, prec, c->stream);
+  if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
+  if (!nocc) return set_error(NOPT_EINVAL, "nopt_jk_build_batch_device: null nocc array");
+  BuildItem it[kMaxBatch];
+  for (int d = 0; d < nd; ++d)
+    it[d] = BuildItem{dD ? dD[d] : nullptr, dC ? dC[d] : nullptr, nocc[d], dJ ? dJ[d] : nullptr,
+                      dK ? dK[d] : nullptr};
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  return enqueue_batch(c, nd, it, prec, s);
+}
+
+int nopt_jk_build_batch(nopt_jk_ctx* c, int nd, const double* const* D, const double* const* C, const long* nocc,
+                        double* const* J, double* const* K, int prec) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
   if (rc) return rc;
-  if (J) HIPCHK(hipMemcpyAsync(J, c->dJ, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
-  if (K) {
-    if (nocc == 0)
-      std::memset(K, 0, sizeof(double) * nn);
-    else
-      HIPCHK(hipMemcpyAsync(K, c->dK, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+  if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
+  if (!nocc) return set_error(NOPT_EINVAL, "nopt_jk_build_batch: null nocc array");
+  const long nn = c->nbf * c->nbf;
+  BuildItem it[kMaxBatch];
+  for (int d = 0; d < nd; ++d) {
+    const double* Dh = D ? D[d] : nullptr;
+    const double* Ch = C ? C[d] : nullptr;
+    double* Jh = J ? J[d] : nullptr;
+    double* Kh = K ? K[d] : nullptr;
+    const long no = nocc[d];
+    if (no < 0 || no > c->nbf) return set_error(NOPT_EINVAL, "nocc out of range");
+    if (Jh && !Dh) return set_error(NOPT_EINVAL, "J requested without a density");
+    if (Kh && no > 0 && !Ch) return set_error(NOPT_EINVAL, "K requested without an orbital factor");
+    // slot d: device staging of D, C and the outputs
+    double** sD = d ? &c->xD[d - 1] : &c->dD;
+    double** sC = d ? &c->xC[d - 1] : &c->dC;
+    double** sJ = d ? &c->xJ[d - 1] : &c->dJ;
+    double** sK = d ? &c->xK[d - 1] : &c->dK;
+    long dcap = nn, jcap = nn, kcap = nn;
+    long* capD = d ? &c->xD_cap[d - 1] : &dcap;
+    long* capC = d ? &c->xC_cap[d - 1] : &c->dC_cap;
+    long* capJ = d ? &c->xJ_cap[d - 1] : &jcap;
+    long* capK = d ? &c->xK_cap[d - 1] : &kcap;
+    if (Dh) {
+      rc = upload(c, sD, capD, Dh, nn);
+      if (rc) return rc;
+    }
+    if (Ch && no > 0) {
+      rc = upload(c, sC, capC, Ch, no * c->nbf);
+      if (rc) return rc;
+    }
+    if (Jh && (*capJ < nn || !*sJ)) {
+      dev_free(*sJ);
+      rc = dev_alloc((void**)sJ, sizeof(double) * nn, "J");
+      if (rc) return rc;
+      *capJ = nn;
+    }
+    if (Kh && (*capK < nn || !*sK)) {
+      dev_free(*sK);
+      rc = dev_alloc((void**)sK, sizeof(double) * nn, "K");
+      if (rc) return rc;
+      *capK = nn;
+    }
+    it[d] = BuildItem{Dh ? *sD : nullptr, (Ch && no > 0) ? *sC : nullptr, no, Jh ? *sJ : nullptr,
+                      Kh ? *sK : nullptr};
+  }
+  rc = enqueue_batch(c, nd, it, prec, c->stream);
+  if (rc) return rc;
+  for (int d = 0; d < nd; ++d) {
+    if (J && J[d]) HIPCHK(hipMemcpyAsync(J[d], it[d].J, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    if (K && K[d]) HIPCHK(hipMemcpyAsync(K[d], it[d].K, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return NOPT_OK;
+}
+
+int nopt_jk_build(nopt_jk_ctx* c, const double* D, const double* C, long nocc, double* J, double* K,
+                  int prec) {
+  return nopt_jk_build_batch(c, 1, &D, &C, &nocc, &J, &K, prec);
 }
 
 int nopt_jk_fock_energy(nopt_jk_ctx* c, const double* H, const double* D, const double* J,

@@ -39,11 +39,11 @@ __device__ __forceinline__ int row_exponent(double mx) {
 }
 
 // Splits the rows of a row-major FP64 matrix (rows_src x cols, ld) into ns
-// int8 digit planes planes[k][row][ldo] (rows_dst rows, zero padded) and row
-// exponents. One wavefront per row.
+// int8 digit planes planes[k*pstride + row*ldo + col] (rows_dst rows, zero
+// padded) and row exponents. One wavefront per row.
 __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ src, long ld, int cols,
                                                    long rows_src, int ns, int8_t* __restrict__ planes,
-                                                   long ldo, long rows_dst, int* __restrict__ expo) {
+                                                   long ldo, long pstride, int* __restrict__ expo) {
   const long row = blockIdx.x;
   const int lane = threadIdx.x;
   const bool live = row < rows_src;
@@ -54,12 +54,11 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   const int e = row_exponent(mx);
   if (lane == 0) expo[row] = e;
-  const long plane = rows_dst * ldo;
   for (int c = lane; c < ldo; c += 64) {
     double r = (live && c < cols) ? ldexp(s[c], 6 - e) : 0.0;  // |r| < 64
     for (int k = 0; k < ns; ++k) {
       const double d = rint(r);
-      planes[k * plane + row * ldo + c] = (int8_t)d;
+      planes[k * pstride + row * ldo + c] = (int8_t)d;
       r = (r - d) * 128.0;
     }
   }
@@ -161,19 +160,23 @@ __global__ __launch_bounds__(256) void k_half_i8(const int8_t* __restrict__ Bd, 
 }
 
 // ------------------------------------------------------------- Coulomb gamma
-// gamma[Q] = sum_mn B[Q][m][n] D[m][n] over the lower triangle (off-diagonal
-// weight 2) with B rebuilt exactly from its five digit planes:
+// gamma[d][Q] = sum_mn B[Q][m][n] D_d[m][n] for ND densities (D_d at
+// Dp + d*dstride), over the lower triangle (off-diagonal weight 2) with B
+// rebuilt exactly from its five digit planes:
 //   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
 // One block per auxiliary function; each wave walks row pairs (p, nbf-1-p)
-// in 8-column chunks.
+// in 8-column chunks, so B is read once for all ND densities.
+template <int ND>
 __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd, long bplane,
                                                   const int* __restrict__ Be, long ld8, int nbf,
-                                                  const double* __restrict__ Dp, long ldd,
-                                                  double* __restrict__ gamma) {
+                                                  const double* __restrict__ Dp, long ldd, long dstride,
+                                                  double* __restrict__ gamma, long gstride) {
   __shared__ double red[4];
   const long Q = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double acc = 0.0;
+  double acc[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) acc[d] = 0.0;
   const int npairs = (nbf + 1) / 2;
   for (int p = w; p < npairs; p += 4) {
     const int m1 = p, m2 = nbf - 1 - p;
@@ -183,25 +186,34 @@ __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd,
       const int n0 = 8 * (ch < c1 ? ch : ch - c1);
       const long row = Q * nbf + m;
       const int8_t* b = Bd + row * ld8 + n0;
-      int8_t d[5][8];
+      int8_t dg[5][8];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(d[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
-      const double* dr = Dp + (long)m * ldd + n0;
-      double s = 0.0;
+      for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
+      double bv[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int n = n0 + j;
-        if (n > m) break;
-        const int hi = 128 * (int)d[0][j] + (int)d[1][j];
-        const int lo = 16384 * (int)d[2][j] + 128 * (int)d[3][j] + (int)d[4][j];
-        const double bv = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
-        s += (n < m ? 2.0 : 1.0) * bv * dr[j];
+        const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
+        const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
+        const double wgt = n < m ? 2.0 : (n == m ? 1.0 : 0.0);
+        bv[j] = wgt * (0x1p-7 * (double)hi + 0x1p-28 * (double)lo);
       }
-      acc += ldexp(s, Be[row] - 6);
+      const int sc = Be[row] - 6;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double* dr = Dp + d * dstride + (long)m * ldd + n0;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += bv[j] * dr[j];
+        acc[d] += ldexp(s, sc);
+      }
     }
   }
-  const double tot = block_sum256(acc, red);
-  if (threadIdx.x == 0) gamma[Q] = tot;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double tot = block_sum256(acc[d], red);
+    if (threadIdx.x == 0) gamma[d * gstride + Q] = tot;
+  }
 }
 
 // ------------------------------------------------------------------ exchange
@@ -339,9 +351,10 @@ __global__ __launch_bounds__(256) void k_syrk_i8(const int8_t* __restrict__ Xd, 
 }  // namespace
 
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                         long rows_dst, int* expo, hipStream_t s) {
+                         long rows_dst, long pstride, int* expo, hipStream_t s) {
   if (rows_dst <= 0) return hipSuccess;
-  k_split_rows<<<(unsigned)rows_dst, 64, 0, s>>>(src, ld, cols, rows_src, ns, planes, ldo, rows_dst, expo);
+  if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo) return hipErrorInvalidValue;
+  k_split_rows<<<(unsigned)rows_dst, 64, 0, s>>>(src, ld, cols, rows_src, ns, planes, ldo, pstride, expo);
   return hipGetLastError();
 }
 
@@ -364,9 +377,15 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
   return hipGetLastError();
 }
 
-hipError_t launch_gamma_i8(int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8, int nbf,
-                           const double* Dp, long ldd, double* gamma) {
-  k_gamma_i8<<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, gamma);
+hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
+                           int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride) {
+  switch (nd) {
+    case 1: k_gamma_i8<1><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
+    case 2: k_gamma_i8<2><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
+    case 3: k_gamma_i8<3><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
+    case 4: k_gamma_i8<4><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

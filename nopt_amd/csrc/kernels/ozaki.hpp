@@ -3,17 +3,19 @@
 #include "common.hpp"
 
 namespace nopt {
-// Rows of src (rows_src x cols, ld) -> ns digit planes planes[k][row][ldo]
-// (rows_dst >= rows_src rows, extra rows and columns zero) and exponents.
+// Rows of src (rows_src x cols, ld) -> ns digit planes
+// planes[k*pstride + row*ldo + col] (rows_dst >= rows_src rows, extra rows and
+// columns zero) and one exponent per row.
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                         long rows_dst, int* expo, hipStream_t s);
+                         long rows_dst, long pstride, int* expo, hipStream_t s);
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
 // (grid: ceil(nbf/64) x naux x column groups).
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                           int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx);
-// gamma_Q = sum_mn B_Qmn D_mn from the five digit planes of B.
-hipError_t launch_gamma_i8(int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8, int nbf,
-                           const double* Dp, long ldd, double* gamma);
+// gamma[d][Q] = sum_mn B_Qmn D_d,mn (d < nd <= 4, D_d at Dp + d*dstride) from
+// the five digit planes of B.
+hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
+                           int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride);
 // Exchange partials over 64x64 lower-triangle tiles (grid: tiles x splits).
 constexpr int kSyrkI8Tile = 64;
 hipError_t launch_syrk_i8(dim3 grid, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,

@@ -5,6 +5,8 @@ HBM. Its methods mirror NOPT's RI layer (include/RI.h:6-23):
 
   set_B(eri3, metric)        gen_RI_AA (src/RI.cpp:34-79), metric folded in
   build(D, C, nocc)          DM_to_F_transform_RI (src/RI.cpp:115-176)
+  build_batch(items)         the two builds of a CASSCF step (src/CAS.cpp:696-707)
+                             with one pass over B
   fock_energy(H, D, J, K)    molecule::calc_F_AO (src/molecule.cpp:4407-4438)
                              and the RHF energy (src/SCF.cpp:134)
   energy_gradient(H, C)      E_RHF(C) and dE/dC = 4 C F (the orbital gradient
@@ -98,6 +100,39 @@ class JKContext:
                                   L.dptr(Jo) if Jo is not None else None,
                                   L.dptr(Ko) if Ko is not None else None, p))
         return Jo, Ko
+
+    def build_batch(self, items, prec="fp64", J=True, K=True):
+        """Builds for several (D, C, nocc) against the same B in one pass
+        (NOPT's inactive + total density pair of a CASSCF step). Each item is
+        (D, C, nocc); D or C may be None. Returns [(J, K), ...]."""
+        n, nd = self.nbf, len(items)
+        if not 1 <= nd <= L.MAX_BATCH:
+            raise ValueError(f"1..{L.MAX_BATCH} densities per batch")
+        p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+        keep, Dp, Cp, no, Jp, Kp, outs = [], [], [], [], [], [], []
+        for D, C, nocc in items:
+            Dm = self._arr(D, (n, n)) if D is not None else None
+            Cm = None
+            if C is not None:
+                Cm = np.ascontiguousarray(C, dtype=np.float64)
+                if Cm.ndim != 2 or Cm.shape[1] != n:
+                    raise ValueError("C must be (norb, nbf) with orbitals as rows")
+                nocc = Cm.shape[0] if nocc is None else nocc
+                if nocc > Cm.shape[0]:
+                    raise ValueError("nocc exceeds the rows of C")
+            Jo = np.empty((n, n)) if (J and Dm is not None) else None
+            Ko = np.empty((n, n)) if K else None
+            keep += [Dm, Cm, Jo, Ko]
+            Dp.append(L.dptr(Dm) if Dm is not None else None)
+            Cp.append(L.dptr(Cm) if Cm is not None else None)
+            no.append(int(nocc or 0))
+            Jp.append(L.dptr(Jo) if Jo is not None else None)
+            Kp.append(L.dptr(Ko) if Ko is not None else None)
+            outs.append((Jo, Ko))
+        arr = lambda v: (L._c_double_p * nd)(*v)
+        L.check(L.lib().nopt_jk_build_batch(self._h, nd, arr(Dp), arr(Cp), (ctypes.c_long * nd)(*no),
+                                            arr(Jp), arr(Kp), p))
+        return outs
 
     def build_device(self, dD, dC, nocc, dJ, dK, prec="fp64", stream=0):
         """Device-pointer build (integer addresses), enqueued on `stream`."""
