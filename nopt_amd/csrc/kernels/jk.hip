@@ -628,6 +628,23 @@ int ensure(void** p, size_t* cap, size_t bytes, const char* what) {
   return rc;
 }
 
+// Column tiling of the integer half-transform: 16*nt columns per group,
+// nt in {2, 4, 6}, with the least padding.
+int nt_for_i8(long ncol, int* ngroups) {
+  const long t = std::max(1L, (ncol + 15) / 16);
+  if (t <= 6) {
+    *ngroups = 1;
+    return t <= 2 ? 2 : t <= 4 ? 4 : 6;
+  }
+  const long g6 = (t + 5) / 6, g4 = (t + 3) / 4;
+  if (4 * g4 < 6 * g6) {
+    *ngroups = (int)g4;
+    return 4;
+  }
+  *ngroups = (int)g6;
+  return 6;
+}
+
 int nt_for(long nocc, int* ngroups) {
   const long t = (nocc + 15) / 16;
   int nt = t <= 1 ? 1 : t <= 2 ? 2 : t <= 4 ? 4 : t <= 6 ? 6 : 8;
@@ -734,7 +751,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   // half-transform (row dots of the B panels already in registers)
   const bool fused = !i8 && nd == 1 && nj == 1 && anyK;
   int ngroups = 1;
-  const int nt = nt_for(anyK ? ncol : 1, &ngroups);
+  const int nt = i8 ? nt_for_i8(anyK ? ncol : 1, &ngroups) : nt_for(anyK ? ncol : 1, &ngroups);
   const long ldx = (long)nt * 16 * ngroups;  // stacked, padded orbital count
   const long ldb = i8 ? c->ld8 : c->ld64;    // leading dimension of the staged D and C
   const int mt = i8 ? c->mtiles64 : c->mtiles;

@@ -11,7 +11,7 @@
 // accumulation adds no rounding at all. Digit pairs of equal weight (k+l)
 // share one int32 accumulator ("class").
 //
-//   half-transform  X = B C^T     4 digits each, pairs k+l <= 3 (10 pairs)
+//   half-transform  X = B C^T     4 digits each, pairs k+l <= 4 (13 pairs)
 //   exchange        K = sum_Q X_Q X_Q^T   4 digits, pairs k+l <= 4 with
 //                   k,l <= 3 (13 pairs); the k+l = 4 terms keep the
 //                   same-number cross terms on the diagonal, which do not
@@ -19,6 +19,8 @@
 //   Coulomb gamma   gamma_Q = sum_mn B_Qmn D_mn from 5 digits of B (35 bits)
 // Class sums are combined in FP64 and scaled by the row exponents.
 #include "ozaki.hpp"
+
+#include <utility>
 
 namespace nopt {
 namespace {
@@ -28,6 +30,16 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ v4i_t mfma_i8(const uint4& a, const uint4& b, v4i_t c) {
   return __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(v4i_t, a), __builtin_bit_cast(v4i_t, b), c,
                                                0, 0, 0);
+}
+
+// f(std::integral_constant<int, i>) for i = 0..N-1, unrolled at compile time
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
 // row maximum -> exponent e with |x| 2^-e < 1 (0 for an all-zero row)
@@ -65,95 +77,146 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
 }
 
 // ------------------------------------------------------------ half-transform
-// X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=3} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
-// in FP64 (the combination of the four exact class sums is exact to FP64
-// rounding). Tile: 64 rows (16 per wave) x 16*NT columns, k-steps of 64.
+// LDS position of 16-byte chunk c (of a 64-byte k-slab) in row r: an XOR
+// swizzle under which the 16 rows x 4 chunks of one MFMA operand read, and
+// the contiguous rows written by the loaders, fall on distinct bank groups for
+// both 32 and 64 four-byte banks.
+__device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >> 3)) & 3); }
+
+// X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=4} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
+// for k, l < 4 (13 digit pairs; the products of equal k+l share one exact
+// int32 accumulator), combined in FP64. Tile: 64 rows x 16*NT columns, each
+// of the four waves owns 16 rows and all columns; k-slabs of 64 bytes, LDS
+// double buffered (one barrier per slab) with the next slab prefetched into
+// registers while the current one is multiplied.
 template <int NT>
-__global__ __launch_bounds__(256) void k_half_i8(const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be,
-                                                 long ld8, int nbf, const int8_t* __restrict__ Cd, long cplane,
-                                                 const int* __restrict__ Ce, double* __restrict__ X, long ldx) {
-  constexpr int BMI = 64, BN = 16 * NT, KS = 64, PAD = 16;
-  __shared__ __attribute__((aligned(16))) int8_t As[4][BMI][KS + PAD];
-  __shared__ __attribute__((aligned(16))) int8_t Cs[4][BN][KS + PAD];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half_i8(
+    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf,
+    const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx) {
+  constexpr int BMI = 64, BN = 16 * NT, KS = 64;
+  constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per buffer
+  constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
+  constexpr int CPT = CCH / 256;                      // = NT
+  static_assert(CCH % 256 == 0, "C slab must split evenly over the block");
+  __shared__ __attribute__((aligned(16))) int8_t sm[2][AB + CB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int mtile = blockIdx.x;
+  const int m0 = blockIdx.x * BMI;
   const long Q = blockIdx.y;
   const int cg = blockIdx.z;
-  const int m0 = mtile * BMI;
-  const int ar = t >> 2, ap = t & 3;  // loader: row ar, digit plane ap (64 bytes)
+  // A loader: row t/4, chunk t%4 of each of the 4 planes
+  const int ar = t >> 2, ac = t & 3;
   const bool arow_ok = m0 + ar < nbf;
-  const int8_t* Ap = Bd + ap * bplane + (Q * nbf + m0 + ar) * ld8;
-  v4i_t acc[4][NT];
+  const int8_t* Ap = Bd + (Q * nbf + m0 + ar) * ld8 + 16 * ac;
+  const int8_t* Cbase = Cd + (long)cg * BN * ld8;
+  // C loader: chunk e = t + 256 u of the slab is chunk (e & 3) of column
+  // (e >> 2) % BN in plane e / (4 BN); global and LDS offsets per u
+  int c_g[CPT], c_s[CPT];
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
+  for (int u = 0; u < CPT; ++u) {
+    const int e = t + 256 * u;
+    const int c = e & 3, col = (e >> 2) % BN, p = e / (4 * BN);
+    c_g[u] = (int)(p * cplane + (long)col * ld8 + 16 * c);
+    c_s[u] = (p * BN + col) * KS + 16 * swz(col, c);
+  }
+  // prefetch registers; the C chunks sit in two arrays of at most 16 dwords
+  // indexed with compile-time constants, so that both stay in registers
+  constexpr int CH = (CPT + 1) / 2;
+  uint4 a_reg[4], c_lo[CH], c_hi[CPT - CH > 0 ? CPT - CH : 1];
+
+#define NOPT_GLOAD_A(k0)                                                                                 \
+  do {                                                                                                   \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p) a_reg[p] =                                             \
+        arow_ok ? *reinterpret_cast<const uint4*>(Ap + p * bplane + (k0)) : uint4{0, 0, 0, 0};           \
+  } while (0)
+#define NOPT_GLOAD_C(k0)                                                                                 \
+  do {                                                                                                   \
+    static_for<CPT>([&](auto U) {                                                                        \
+      constexpr int u_ = decltype(U)::value;                                                             \
+      const uint4 v_ = *reinterpret_cast<const uint4*>(Cbase + c_g[u_] + (k0));                          \
+      if constexpr (u_ < CH) c_lo[u_ < CH ? u_ : 0] = v_;                                                \
+      else c_hi[u_ < CH ? 0 : u_ - CH] = v_;                                                             \
+    });                                                                                                  \
+  } while (0)
+#define NOPT_LSTORE(buf)                                                                                 \
+  do {                                                                                                   \
+    int8_t* As_ = sm[buf];                                                                               \
+    _Pragma("unroll") for (int p = 0; p < 4; ++p)                                                        \
+        *reinterpret_cast<uint4*>(As_ + (p * BMI + ar) * KS + 16 * swz(ar, ac)) = a_reg[p];              \
+    static_for<CPT>([&](auto U) {                                                                        \
+      constexpr int u_ = decltype(U)::value;                                                             \
+      uint4* d_ = reinterpret_cast<uint4*>(As_ + AB + c_s[u_]);                                          \
+      if constexpr (u_ < CH) *d_ = c_lo[u_ < CH ? u_ : 0];                                               \
+      else *d_ = c_hi[u_ < CH ? 0 : u_ - CH];                                                            \
+    });                                                                                                  \
+  } while (0)
+  v4i_t acc[NT][5];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) acc[c][j] = v4i_t{0, 0, 0, 0};
-  uint4 a_reg[4];
-  constexpr int CCH = BN * 4 * (KS / 16);  // 16-byte chunks of the C tile
-  constexpr int CPT = (CCH + 255) / 256;
-  uint4 c_reg[CPT];
-  auto gload = [&](int k0) {
+  for (int j = 0; j < NT; ++j)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-      a_reg[i] = arow_ok ? *reinterpret_cast<const uint4*>(Ap + k0 + 16 * i) : uint4{0, 0, 0, 0};
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-      const int e = t + 256 * u;
-      if (e < CCH) {
-        const int col = e / 16, p = (e / 4) % 4, seg = (e % 4) * 16;
-        c_reg[u] = *reinterpret_cast<const uint4*>(Cd + p * cplane + ((long)cg * BN + col) * ld8 + k0 + seg);
-      }
-    }
-  };
-  gload(0);
-  for (int k0 = 0; k0 < ld8; k0 += KS) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<uint4*>(&As[ap][ar][16 * i]) = a_reg[i];
-#pragma unroll
-    for (int u = 0; u < CPT; ++u) {
-      const int e = t + 256 * u;
-      if (e < CCH) {
-        const int col = e / 16, p = (e / 4) % 4, seg = (e % 4) * 16;
-        *reinterpret_cast<uint4*>(&Cs[p][col][seg]) = c_reg[u];
-      }
-    }
-    __syncthreads();
-    if (k0 + KS < ld8) gload(k0 + KS);
+    for (int c = 0; c < 5; ++c) acc[j][c] = v4i_t{0, 0, 0, 0};
+  const int nks = (int)(ld8 / KS);
+  NOPT_GLOAD_A(0);
+  NOPT_GLOAD_C(0);
+  NOPT_LSTORE(0);
+  __syncthreads();
+  const int lr = lane & 15, lc = lane >> 4;
+  const int r = w * 16 + lr;
+  for (int ks = 0; ks < nks; ++ks) {
+    const int cur = ks & 1;
+    // the B slab (HBM) is fetched before the products, the small C slab
+    // (L2-resident) after them, which keeps its registers free meanwhile
+    if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
+    const int8_t* As = sm[cur];
+    const int8_t* Cs = sm[cur] + AB;
     uint4 av[4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
-      av[p] = *reinterpret_cast<const uint4*>(&As[p][w * 16 + lane % 16][16 * (lane / 16)]);
+    for (int p = 0; p < 4; ++p) av[p] = *reinterpret_cast<const uint4*>(As + (p * BMI + r) * KS + 16 * swz(r, lc));
+    // one digit plane l of the C operand at a time: pairs (k, l), k+l <= 4
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      uint4 bv[4];
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
-        bv[p] = *reinterpret_cast<const uint4*>(&Cs[p][j * 16 + lane % 16][16 * (lane / 16)]);
-      acc[0][j] = mfma_i8(av[0], bv[0], acc[0][j]);
-      acc[1][j] = mfma_i8(av[0], bv[1], acc[1][j]);
-      acc[1][j] = mfma_i8(av[1], bv[0], acc[1][j]);
-      acc[2][j] = mfma_i8(av[0], bv[2], acc[2][j]);
-      acc[2][j] = mfma_i8(av[1], bv[1], acc[2][j]);
-      acc[2][j] = mfma_i8(av[2], bv[0], acc[2][j]);
-      acc[3][j] = mfma_i8(av[0], bv[3], acc[3][j]);
-      acc[3][j] = mfma_i8(av[1], bv[2], acc[3][j]);
-      acc[3][j] = mfma_i8(av[2], bv[1], acc[3][j]);
-      acc[3][j] = mfma_i8(av[3], bv[0], acc[3][j]);
+      const int col = j * 16 + lr;
+      const int8_t* cp = Cs + col * KS + 16 * swz(col, lc);
+      v4i_t* A = acc[j];
+      uint4 bv = *reinterpret_cast<const uint4*>(cp);
+      A[0] = mfma_i8(av[0], bv, A[0]);
+      A[1] = mfma_i8(av[1], bv, A[1]);
+      A[2] = mfma_i8(av[2], bv, A[2]);
+      A[3] = mfma_i8(av[3], bv, A[3]);
+      bv = *reinterpret_cast<const uint4*>(cp + BN * KS);
+      A[1] = mfma_i8(av[0], bv, A[1]);
+      A[2] = mfma_i8(av[1], bv, A[2]);
+      A[3] = mfma_i8(av[2], bv, A[3]);
+      A[4] = mfma_i8(av[3], bv, A[4]);
+      bv = *reinterpret_cast<const uint4*>(cp + 2 * BN * KS);
+      A[2] = mfma_i8(av[0], bv, A[2]);
+      A[3] = mfma_i8(av[1], bv, A[3]);
+      A[4] = mfma_i8(av[2], bv, A[4]);
+      bv = *reinterpret_cast<const uint4*>(cp + 3 * BN * KS);
+      A[3] = mfma_i8(av[0], bv, A[3]);
+      A[4] = mfma_i8(av[1], bv, A[4]);
     }
+    if (ks + 1 < nks) {
+      NOPT_GLOAD_C((ks + 1) * KS);
+      NOPT_LSTORE(cur ^ 1);
+    }
+    __syncthreads();
   }
+#undef NOPT_GLOAD_A
+#undef NOPT_GLOAD_C
+#undef NOPT_LSTORE
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    const int m = m0 + w * 16 + 4 * (lane / 16) + v;
+    const int m = m0 + w * 16 + 4 * lc + v;
     if (m < nbf) {
       const int em = Be[Q * nbf + m];
-      double* xr = X + ((long)Q * nbf + m) * ldx + (long)cg * BN + lane % 16;
+      double* xr = X + ((long)Q * nbf + m) * ldx;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
-        const int col = cg * BN + j * 16 + lane % 16;
-        const double s = (double)acc[0][j][v] + 0x1p-7 * (double)acc[1][j][v] + 0x1p-14 * (double)acc[2][j][v] +
-                         0x1p-21 * (double)acc[3][j][v];
-        xr[j * 16] = ldexp(s, em + Ce[col] - 12);
+        const int col = cg * BN + j * 16 + lr;
+        const v4i_t* A = acc[j];
+        const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
+                          0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
+        xr[col] = ldexp(sv, em + Ce[col] - 12);
       }
     }
   }
@@ -360,20 +423,13 @@ hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, in
 
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                           int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx) {
-#define NOPT_HI8(NTV)                                                                                        \
-  case NTV:                                                                                                  \
-    k_half_i8<NTV><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx);                 \
-    break;
+  if (ld8 % 64 != 0 || (long)grid.z * 16 * nt > ldx) return hipErrorInvalidValue;
   switch (nt) {
-    NOPT_HI8(1)
-    NOPT_HI8(2)
-    NOPT_HI8(4)
-    NOPT_HI8(6)
-    NOPT_HI8(8)
-    default:
-      return hipErrorInvalidValue;
+    case 2: k_half_i8<2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
+    case 4: k_half_i8<4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
+    case 6: k_half_i8<6><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
+    default: return hipErrorInvalidValue;
   }
-#undef NOPT_HI8
   return hipGetLastError();
 }
 

@@ -9,7 +9,8 @@ namespace nopt {
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
                          long rows_dst, long pstride, int* expo, hipStream_t s);
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
-// (grid: ceil(nbf/64) x naux x column groups).
+// (grid: ceil(nbf/64) x naux x column groups of 16*nt columns, nt in {2,4,6};
+// ld8 a multiple of 64).
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                           int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx);
 // gamma[d][Q] = sum_mn B_Qmn D_d,mn (d < nd <= 4, D_d at Dp + d*dstride) from
