@@ -759,11 +759,20 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   const long rows = c->naux * c->nbf;
   const long nn = (long)nbf * nbf;
   int rc;
+  // integer mode with one column group: the half-transform writes the digit
+  // planes of X directly (per density), X itself is never stored
+  const bool xfused = i8 && anyK && ngroups == 1;
+  long xoff[kMaxBatch + 1];  // digit-plane offsets (bytes) of each density
+  xoff[0] = 0;
+  for (int d = 0; d < nd; ++d)
+    xoff[d + 1] = xoff[d] + ((off[d + 1] > off[d]) ? 4 * rows * round_up(it[d].nocc, 64) : 0);
   if (anyK) {
-    size_t cap = c->X_bytes;
-    rc = ensure(&c->X, &cap, esz * rows * ldx, "half-transformed tensor X");
-    if (rc) return rc;
-    c->X_bytes = cap;
+    if (!xfused) {
+      size_t cap = c->X_bytes;
+      rc = ensure(&c->X, &cap, esz * rows * ldx, "half-transformed tensor X");
+      if (rc) return rc;
+      c->X_bytes = cap;
+    }
     size_t cpb = c->Cp_bytes;
     rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
     if (rc) return rc;
@@ -778,9 +787,13 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       long ldk_max = 0;
       for (int d = 0; d < nd; ++d)
         if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, round_up(it[d].nocc, 64));
+      // fused: planes of every density side by side; otherwise one density
+      // at a time
+      const size_t xdb = xfused ? (size_t)xoff[nd] : (size_t)4 * rows * ldk_max;
+      const size_t xeb = sizeof(int) * rows * (xfused ? nd : 1);
       size_t b3 = c->Xd_bytes, b4 = c->Xe_bytes;
-      rc = ensure(&c->Xd, &b3, (size_t)4 * rows * ldk_max, "exchange digit planes");
-      if (!rc) rc = ensure(&c->Xe, &b4, sizeof(int) * rows, "exchange row exponents");
+      rc = ensure(&c->Xd, &b3, xdb, "exchange digit planes");
+      if (!rc) rc = ensure(&c->Xe, &b4, xeb, "exchange row exponents");
       if (rc) return rc;
       c->Xd_bytes = b3;
       c->Xe_bytes = b4;
@@ -830,9 +843,23 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   if (anyK) {
     dim3 gh(mt, naux, ngroups);
     hipError_t e;
-    if (i8)
+    if (i8) {
+      nopt::XSplit xs{};
+      if (xfused)
+        for (int d = 0; d < nd; ++d) {
+          if (off[d + 1] == off[d]) continue;
+          const int k = xs.nseg++;
+          const long ldk = round_up(it[d].nocc, 64);
+          xs.off[k] = (int)off[d];
+          xs.len[k] = (int)it[d].nocc;
+          xs.ldk[k] = ldk;
+          xs.xplane[k] = rows * ldk;
+          xs.xd[k] = (int8_t*)c->Xd + xoff[d];
+          xs.xe[k] = (int*)c->Xe + (long)d * rows;
+        }
       e = nopt::launch_half_i8(nt, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
-                               ldx * c->ld8, (const int*)c->Ce, (double*)c->X, ldx);
+                               ldx * c->ld8, (const int*)c->Ce, xfused ? nullptr : (double*)c->X, ldx, xs);
+    }
     else if (f64)
       e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X, ldx,
                               c->gpart, c->mtiles);
@@ -879,10 +906,12 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     const unsigned nel = (unsigned)((nn + 255) / 256);
     if (i8) {
       const long ldk = round_up(it[d].nocc, 64);
-      HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)c->Xd, ldk,
-                                 rows, rows * ldk, (int*)c->Xe, s));
-      HIPCHK(nopt::launch_syrk_i8(dim3(c->ntiles8, c->nsk8), s, (const int8_t*)c->Xd, rows * ldk,
-                                  (const int*)c->Xe, ldk, nbf, naux, c->Kpart));
+      const int8_t* xd = (const int8_t*)c->Xd + (xfused ? xoff[d] : 0);
+      const int* xe = (const int*)c->Xe + (xfused ? (long)d * rows : 0);
+      if (!xfused)
+        HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
+                                   rows, rows * ldk, (int*)xe, s));
+      HIPCHK(nopt::launch_syrk_i8(dim3(c->ntiles8, c->nsk8), s, xd, rows * ldk, xe, ldk, nbf, naux, c->Kpart));
       k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
                                      it[d].K);
     } else {
@@ -954,9 +983,27 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
   c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
   {
+    // integer exchange: tiles x Q-splits sized to whole rounds of resident
+    // blocks (two per CU), so that the last round is not left mostly idle
     const int kt8 = (int)((nbf + nopt::kSyrkI8Tile - 1) / nopt::kSyrkI8Tile);
     c->ntiles8 = kt8 * (kt8 + 1) / 2;
-    c->nsk8 = std::max(1, std::min((int)naux, (2048 + c->ntiles8 - 1) / c->ntiles8));
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+      ncu = 256;
+    const long slots = 2L * ncu;
+    int best = 1;
+    double best_eff = 0.0;
+    for (int r = 1; r <= 8; ++r) {
+      const int nsp = (int)std::max(1L, std::min((long)naux, slots * r / c->ntiles8));
+      const long blocks = (long)nsp * c->ntiles8;
+      const double eff = (double)blocks / (double)(slots * ((blocks + slots - 1) / slots));
+      if (eff > best_eff + 1e-9) {
+        best_eff = eff;
+        best = nsp;
+      }
+      if (eff >= 0.9) break;
+    }
+    c->nsk8 = best;
   }
   int rc = NOPT_OK;
   hipError_t e = hipSetDevice(device);

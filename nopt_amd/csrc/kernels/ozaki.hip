@@ -92,7 +92,8 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 template <int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half_i8(
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf,
-    const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx) {
+    const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
+    const XSplit xs) {
   constexpr int BMI = 64, BN = 16 * NT, KS = 64;
   constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per buffer
   constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
@@ -204,21 +205,83 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #undef NOPT_GLOAD_A
 #undef NOPT_GLOAD_C
 #undef NOPT_LSTORE
+  if (xs.nseg == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int m = m0 + w * 16 + 4 * lc + v;
+      if (m < nbf) {
+        const int em = Be[Q * nbf + m];
+        double* xr = X + ((long)Q * nbf + m) * ldx;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const int col = cg * BN + j * 16 + lr;
+          const v4i_t* A = acc[j];
+          const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
+                            0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
+          xr[col] = ldexp(sv, em + Ce[col] - 12);
+        }
+      }
+    }
+    return;
+  }
+  // fused digit split (one column group): X in FP64 registers, then per
+  // segment a row maximum over the segment's columns (16 lanes share a row),
+  // digits staged in LDS and written out as 16-byte chunks
+  double xv[NT][4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int m = m0 + w * 16 + 4 * lc + v;
-    if (m < nbf) {
-      const int em = Be[Q * nbf + m];
-      double* xr = X + ((long)Q * nbf + m) * ldx;
+    const int em = m < nbf ? Be[Q * nbf + m] : 0;
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const int col = cg * BN + j * 16 + lr;
-        const v4i_t* A = acc[j];
-        const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
-                          0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
-        xr[col] = ldexp(sv, em + Ce[col] - 12);
-      }
+    for (int j = 0; j < NT; ++j) {
+      const v4i_t* A = acc[j];
+      const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
+                        0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
+      xv[j][v] = m < nbf ? ldexp(sv, em + Ce[j * 16 + lr] - 12) : 0.0;
     }
+  }
+  int8_t* st = &sm[0][0];
+  for (int d = 0; d < xs.nseg; ++d) {
+    const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
+    const int ldk = (int)xs.ldk[d];
+    const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
+    for (int i = t; i < nch; i += 256) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = w * 16 + 4 * lc + v;
+      const int m = m0 + row;
+      double mx = 0.0;
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        if (j * 16 >= o && j * 16 < o + pad) mx = fmax(mx, fabs(xv[j][v]));
+#pragma unroll
+      for (int sh = 1; sh < 16; sh <<= 1) mx = fmax(mx, __shfl_xor(mx, sh, 64));
+      const int e = row_exponent(mx);
+      if (lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        if (j * 16 >= o && j * 16 < o + pad) {
+          double r = ldexp(xv[j][v], 6 - e);
+          const int cidx = j * 16 + lr - o;
+#pragma unroll
+          for (int p = 0; p < 4; ++p) {
+            const double dg = rint(r);
+            st[(p * BMI + row) * ldk + cidx] = (int8_t)dg;
+            r = (r - dg) * 128.0;
+          }
+        }
+    }
+    __syncthreads();
+    const int cpr = ldk / 16;  // chunks per row
+    for (int i = t; i < nch; i += 256) {
+      const int p = i / (BMI * cpr), rem = i % (BMI * cpr), row = rem / cpr, ch = rem % cpr;
+      const int m = m0 + row;
+      if (m < nbf)
+        *reinterpret_cast<uint4*>(xs.xd[d] + p * xs.xplane[d] + (Q * nbf + m) * (long)ldk + 16 * ch) =
+            reinterpret_cast<const uint4*>(st)[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -287,126 +350,176 @@ __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
   bj = t - bi * (bi + 1) / 2;
 }
 
-// Kpart[s][tile][r][c] = sum_{Q in split s} 2^(e_Qm + e_Qn - 12) sum_{k,l}
-//   2^(-7(k+l)) <Xd_k[Q][m][:], Xd_l[Q][n][:]>,  m = 64 bi + r, n = 64 bj + c.
-// Tile 64x64: 4 waves of 32x32 (2x2 MFMA blocks), k-steps of 64 over the
-// padded orbital index.
+// Kpart[s][tile][r][c] = sum_{Q in split s} 2^(e_Qm + e_Qn - 12) sum_{k+l<=4}
+//   2^(-7(k+l)) <Xd_k[Q][m][:], Xd_l[Q][n][:]>,  m = 64 bi + r, n = 64 bj + c,
+// k, l < 4 (13 digit pairs in 5 exact int32 classes, combined in FP64 once
+// per auxiliary function). Tile 64x64: 4 waves of 32x32 (2x2 MFMA blocks);
+// the operands stream as 64-byte k-slabs over (Q, orbital chunk), double
+// buffered in swizzled LDS with one barrier per slab; each thread prefetches
+// the row exponents of its own accumulator rows and columns.
 constexpr int KT8 = 64;
-__global__ __launch_bounds__(256) void k_syrk_i8(const int8_t* __restrict__ Xd, long xplane,
-                                                 const int* __restrict__ Xe, long ldk, int nbf, int naux,
-                                                 double* __restrict__ Kpart) {
-  constexpr int KS = 64, PAD = 16;
-  __shared__ __attribute__((aligned(16))) int8_t As[4][KT8][KS + PAD];
-  __shared__ __attribute__((aligned(16))) int8_t Bs[4][KT8][KS + PAD];
-  __shared__ int ea[KT8], eb[KT8];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = w >> 1, wc = w & 1;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_syrk_i8(
+    const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
+    double* __restrict__ Kpart) {
+  constexpr int KS = 64, TB = 4 * KT8 * KS;  // bytes of one operand tile (4 planes)
+  __shared__ __attribute__((aligned(16))) int8_t sm[2][2 * TB];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
+  const int lr = lane & 15, lc = lane >> 4;
   int bi, bj;
   tri_tile(blockIdx.x, bi, bj);
-  const int s = blockIdx.y, ns = gridDim.y;
-  const int q0 = (int)((long)naux * s / ns), q1 = (int)((long)naux * (s + 1) / ns);
-  // loader role: row lr of both blocks, digit plane lp (64 bytes per k-step)
-  const int lr = t >> 2, lp = t & 3;
-  const bool ra = bi * KT8 + lr < nbf, rb = bj * KT8 + lr < nbf;
-  v4i_t acc[5][2][2];
+  const int sp = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
+  const int nkq = (int)(ldk / KS);  // k-slabs per auxiliary function
+  const int nsl = (q1 - q0) * nkq;
+  // loader: row t/4 of each operand tile, chunk t%4, all 4 planes
+  const int ar = t >> 2, ac = t & 3;
+  const int ma = bi * KT8 + ar, mb = bj * KT8 + ar;
+  const bool ra = ma < nbf, rb = mb < nbf;
+  const int8_t* pa = Xd + (long)ma * ldk + 16 * ac;
+  const int8_t* pb = Xd + (long)mb * ldk + 16 * ac;
+  const long qstride = (long)nbf * ldk;
+  uint4 a_reg[4], b_reg[4];
+  auto gload = [&](int sl) {
+    const int q = q0 + sl / nkq, k0 = (sl % nkq) * KS;
+    const long off = (long)q * qstride + k0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      a_reg[p] = ra ? *reinterpret_cast<const uint4*>(pa + p * xplane + off) : uint4{0, 0, 0, 0};
+      b_reg[p] = rb ? *reinterpret_cast<const uint4*>(pb + p * xplane + off) : uint4{0, 0, 0, 0};
+    }
+  };
+  auto lstore = [&](int buf) {
+    int8_t* As = sm[buf];
+    int8_t* Bs = sm[buf] + TB;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      *reinterpret_cast<uint4*>(As + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = a_reg[p];
+      *reinterpret_cast<uint4*>(Bs + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = b_reg[p];
+    }
+  };
+  // exponents of this thread's accumulator rows (2 x 4) and columns (2)
+  int ea[2][4], eb[2], na[2][4], nb[2];
+  auto eload = [&](int q, int (&xa)[2][4], int (&xb)[2]) {
+    const int* e = Xe + (long)q * nbf;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = bi * KT8 + wr * 32 + i * 16 + 4 * lc + v;
+        xa[i][v] = m < nbf ? e[m] : 0;
+      }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = bj * KT8 + wc * 32 + j * 16 + lr;
+      xb[j] = n < nbf ? e[n] : 0;
+    }
+  };
+  v4i_t acc[2][2][5];
   d4_t k64[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) k64[i][j] = d4_t{0, 0, 0, 0};
-  uint4 a_reg[4], b_reg[4];
-  auto gload = [&](int q, int k0) {
-    const int8_t* pa = Xd + lp * xplane + ((long)q * nbf + bi * KT8 + lr) * ldk + k0;
-    const int8_t* pb = Xd + lp * xplane + ((long)q * nbf + bj * KT8 + lr) * ldk + k0;
+    for (int j = 0; j < 2; ++j) {
+      k64[i][j] = d4_t{0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a_reg[i] = ra ? *reinterpret_cast<const uint4*>(pa + 16 * i) : uint4{0, 0, 0, 0};
-      b_reg[i] = rb ? *reinterpret_cast<const uint4*>(pb + 16 * i) : uint4{0, 0, 0, 0};
+      for (int c = 0; c < 5; ++c) acc[i][j][c] = v4i_t{0, 0, 0, 0};
     }
-  };
-  if (q0 < q1) gload(q0, 0);
-  for (int q = q0; q < q1; ++q) {
+  if (nsl > 0) {
+    gload(0);
+    eload(q0, ea, eb);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int sl = 0; sl < nsl; ++sl) {
+    const int cur = sl & 1;
+    const bool more = sl + 1 < nsl;
+    const bool qend = (sl % nkq) == nkq - 1;
+    if (more) gload(sl + 1);
+    if (more && qend) eload(q0 + (sl + 1) / nkq, na, nb);
+    const int8_t* As = sm[cur];
+    const int8_t* Bs = sm[cur] + TB;
+    uint4 av[2][4];
 #pragma unroll
-    for (int c = 0; c < 5; ++c)
+    for (int i = 0; i < 2; ++i) {
+      const int r = wr * 32 + i * 16 + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * KT8 + r) * KS + 16 * swz(r, lc));
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int cidx = wc * 32 + j * 16 + lr;
+      const int8_t* cp = Bs + cidx * KS + 16 * swz(cidx, lc);
+      uint4 bv = *reinterpret_cast<const uint4*>(cp);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i_t* A = acc[i][j];
+        A[0] = mfma_i8(av[i][0], bv, A[0]);
+        A[1] = mfma_i8(av[i][1], bv, A[1]);
+        A[2] = mfma_i8(av[i][2], bv, A[2]);
+        A[3] = mfma_i8(av[i][3], bv, A[3]);
+      }
+      bv = *reinterpret_cast<const uint4*>(cp + KT8 * KS);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i_t* A = acc[i][j];
+        A[1] = mfma_i8(av[i][0], bv, A[1]);
+        A[2] = mfma_i8(av[i][1], bv, A[2]);
+        A[3] = mfma_i8(av[i][2], bv, A[3]);
+        A[4] = mfma_i8(av[i][3], bv, A[4]);
+      }
+      bv = *reinterpret_cast<const uint4*>(cp + 2 * KT8 * KS);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i_t* A = acc[i][j];
+        A[2] = mfma_i8(av[i][0], bv, A[2]);
+        A[3] = mfma_i8(av[i][1], bv, A[3]);
+        A[4] = mfma_i8(av[i][2], bv, A[4]);
+      }
+      bv = *reinterpret_cast<const uint4*>(cp + 3 * KT8 * KS);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        v4i_t* A = acc[i][j];
+        A[3] = mfma_i8(av[i][0], bv, A[3]);
+        A[4] = mfma_i8(av[i][1], bv, A[4]);
+      }
+    }
+    if (qend) {
+      // combine the class sums of this auxiliary function in FP64
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[c][i][j] = v4i_t{0, 0, 0, 0};
-    for (int k0 = 0; k0 < ldk; k0 += KS) {
-      __syncthreads();
+        for (int j = 0; j < 2; ++j) {
+          v4i_t* A = acc[i][j];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        *reinterpret_cast<uint4*>(&As[lp][lr][16 * i]) = a_reg[i];
-        *reinterpret_cast<uint4*>(&Bs[lp][lr][16 * i]) = b_reg[i];
-      }
-      if (k0 == 0 && t < KT8) {
-        ea[t] = bi * KT8 + t < nbf ? Xe[(long)q * nbf + bi * KT8 + t] : 0;
-        eb[t] = bj * KT8 + t < nbf ? Xe[(long)q * nbf + bj * KT8 + t] : 0;
-      }
-      __syncthreads();
-      {
-        int nq = q, nk = k0 + KS;
-        if (nk >= ldk) {
-          nk = 0;
-          ++nq;
+          for (int v = 0; v < 4; ++v) {
+            const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
+                              0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
+            k64[i][j][v] += ldexp(sv, ea[i][v] + eb[j] - 12);
+          }
+#pragma unroll
+          for (int c = 0; c < 5; ++c) A[c] = v4i_t{0, 0, 0, 0};
         }
-        if (nq < q1) gload(nq, nk);
-      }
-      uint4 av[4][2];
-#pragma unroll
-      for (int p = 0; p < 4; ++p)
+      if (more) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-          av[p][i] = *reinterpret_cast<const uint4*>(&As[p][wr * 32 + i * 16 + lane % 16][16 * (lane / 16)]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        uint4 bv[4];
-#pragma unroll
-        for (int p = 0; p < 4; ++p)
-          bv[p] = *reinterpret_cast<const uint4*>(&Bs[p][wc * 32 + j * 16 + lane % 16][16 * (lane / 16)]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          acc[0][i][j] = mfma_i8(av[0][i], bv[0], acc[0][i][j]);
-          acc[1][i][j] = mfma_i8(av[0][i], bv[1], acc[1][i][j]);
-          acc[1][i][j] = mfma_i8(av[1][i], bv[0], acc[1][i][j]);
-          acc[2][i][j] = mfma_i8(av[0][i], bv[2], acc[2][i][j]);
-          acc[2][i][j] = mfma_i8(av[1][i], bv[1], acc[2][i][j]);
-          acc[2][i][j] = mfma_i8(av[2][i], bv[0], acc[2][i][j]);
-          acc[3][i][j] = mfma_i8(av[0][i], bv[3], acc[3][i][j]);
-          acc[3][i][j] = mfma_i8(av[1][i], bv[2], acc[3][i][j]);
-          acc[3][i][j] = mfma_i8(av[2][i], bv[1], acc[3][i][j]);
-          acc[3][i][j] = mfma_i8(av[3][i], bv[0], acc[3][i][j]);
-          acc[4][i][j] = mfma_i8(av[1][i], bv[3], acc[4][i][j]);
-          acc[4][i][j] = mfma_i8(av[2][i], bv[2], acc[4][i][j]);
-          acc[4][i][j] = mfma_i8(av[3][i], bv[1], acc[4][i][j]);
-        }
+          for (int v = 0; v < 4; ++v) ea[i][v] = na[i][v];
+        eb[0] = nb[0];
+        eb[1] = nb[1];
       }
     }
-    // combine the class sums of this auxiliary function in FP64
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int cc = wc * 32 + j * 16 + lane % 16;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int rr = wr * 32 + i * 16 + 4 * (lane / 16) + v;
-          const double sv = (double)acc[0][i][j][v] + 0x1p-7 * (double)acc[1][i][j][v] +
-                            0x1p-14 * (double)acc[2][i][j][v] + 0x1p-21 * (double)acc[3][i][j][v] +
-                            0x1p-28 * (double)acc[4][i][j][v];
-          k64[i][j][v] += ldexp(sv, ea[rr] + eb[cc] - 12);
-        }
-      }
+    if (more) lstore(cur ^ 1);
+    __syncthreads();
   }
-  double* Kt = Kpart + ((long)s * gridDim.x + blockIdx.x) * KT8 * KT8;
+  double* Kt = Kpart + ((long)sp * gridDim.x + blockIdx.x) * KT8 * KT8;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int rr = wr * 32 + i * 16 + 4 * (lane / 16) + v;
-        const int cc = wc * 32 + j * 16 + lane % 16;
+        const int rr = wr * 32 + i * 16 + 4 * lc + v;
+        const int cc = wc * 32 + j * 16 + lr;
         Kt[rr * KT8 + cc] = k64[i][j][v];
       }
 }
@@ -422,12 +535,18 @@ hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, in
 }
 
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx) {
+                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx,
+                          const XSplit& xs) {
   if (ld8 % 64 != 0 || (long)grid.z * 16 * nt > ldx) return hipErrorInvalidValue;
+  if (xs.nseg < 0 || xs.nseg > 4 || (xs.nseg > 0 && grid.z != 1)) return hipErrorInvalidValue;
+  for (int d = 0; d < xs.nseg; ++d)
+    if (xs.off[d] % 16 || xs.len[d] <= 0 || xs.ldk[d] % 64 || xs.ldk[d] > 128 || xs.ldk[d] < xs.len[d] ||
+        xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] || !xs.xe[d])
+      return hipErrorInvalidValue;
   switch (nt) {
-    case 2: k_half_i8<2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
-    case 4: k_half_i8<4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
-    case 6: k_half_i8<6><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx); break;
+    case 2: k_half_i8<2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 4: k_half_i8<4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 6: k_half_i8<6><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

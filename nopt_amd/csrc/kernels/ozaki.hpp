@@ -8,11 +8,27 @@ namespace nopt {
 // columns zero) and one exponent per row.
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
                          long rows_dst, long pstride, int* expo, hipStream_t s);
+// Digit split of the half-transformed rows fused into the half-transform:
+// segment d covers the columns [off[d], off[d] + len[d]) of X (off a multiple
+// of 16); its rows (Q, m) get their own exponent xe[d][Q*nbf + m] and four
+// digit planes xd[d][p*xplane[d] + (Q*nbf + m)*ldk[d] + (col - off[d])],
+// zero beyond len[d] (ldk[d] a multiple of 64, at most 128).
+struct XSplit {
+  int nseg;
+  int off[4], len[4];
+  long ldk[4], xplane[4];
+  int8_t* xd[4];
+  int* xe[4];
+};
+
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
 // (grid: ceil(nbf/64) x naux x column groups of 16*nt columns, nt in {2,4,6};
 // ld8 a multiple of 64).
+// With xs.nseg > 0 (one column group only) the rows are split into digit
+// planes in the epilogue and X is not written.
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx);
+                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx,
+                          const XSplit& xs);
 // gamma[d][Q] = sum_mn B_Qmn D_d,mn (d < nd <= 4, D_d at Dp + d*dstride) from
 // the five digit planes of B.
 hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
