@@ -6,12 +6,14 @@ Workload: all-trans C20H42, cc-pVDZ (490 spherical functions) / cc-pVDZ-JKFIT
 density, K from the occupied orbital factor) of the converged RI-RHF density,
 with B = L^-1 (P|mn) resident in HBM and the inputs already on the device.
 
-Every precision mode is timed (J is FP64 in all of them): fp64, fp32 (FP32
-MFMA products, FP64 accumulation), bf16x6 (three-piece bf16 split of FP32
-operands, six MFMA products, FP64 accumulation) and bf16x3. The headline
+Every precision mode is timed (J is FP64 in all of them): fp64, int8 (int8
+digit splits with power-of-two row scales, exact int32 MFMA sums), fp32
+(FP32 MFMA products, FP64 accumulation), bf16x6 (three-piece bf16 split of
+FP32 operands, six MFMA products, FP64 accumulation) and bf16x3. The headline
 value is the fastest mode whose energy error against the FP64 build on the
 same density stays below 1 micro-Hartree (--prec auto), next to the FP64
-build's own rate.
+build's own rate. Accuracy is also reported on ten perturbed densities
+D + 1e-3 sym(R) (seed 0), whose K comes from a signed eigen-factor.
 
 The CPU baseline times the oracle's restatement of NOPT's
 DM_to_F_transform_RI (Cartesian AOs, raw (mn|P), V^-1/2 applied per build,
@@ -28,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"hbm_GBps": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3, "bf16_TF": 2500.0}
+PEAK = {"hbm_GBps": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3, "bf16_TF": 2500.0, "int8_TOPS": 5000.0}
 
 
 def log(*a):
@@ -59,34 +61,84 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
     return steps / dt, 1000.0 * dt / steps, phases
 
 
+# kernels of each timed phase, per mode
+KERNELS = {
+    "half_transform": {"fp64": "k_half<double>", "fp32": "k_half<float>", "bf16x6": "k_half_bf16<3>",
+                       "bf16x3": "k_half_bf16<2>", "int8": "k_half_i8"},
+    "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
+              "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)", "int8": "k_gamma_i8"},
+    "coulomb_j": {p: "k_jpart<1>" for p in ("fp64", "fp32", "bf16x6", "bf16x3", "int8")},
+    "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
+                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8"},
+}
+
+
 def roofline(phase, ms, nbf, naux, nocc, prec):
-    """Algorithmic work of one launch of the dominant kernel."""
+    """Algorithmic work of the phase's kernel per launch, over its HIP-event time.
+    Bytes count each operand element once at its stored width (no padding);
+    int8 counts 13 digit products per FP64 product."""
     t = ms * 1e-3
+    npair = nbf * (nbf + 1) / 2
+
+    def hbm(byts):
+        a = byts / t / 1e9
+        return dict(bound="hbm", achieved=a, peak=PEAK["hbm_GBps"], unit="GB/s", frac=a / PEAK["hbm_GBps"],
+                    traffic=None, kernel=KERNELS[phase][prec], algorithmic_bytes=byts)
+
+    def mfma(ops, peak, unit="TFLOP/s"):
+        a = ops / t / 1e12
+        return dict(bound="mfma", achieved=a, peak=peak, unit=unit, frac=a / peak, traffic=None,
+                    kernel=KERNELS[phase][prec], algorithmic_flops=ops)
+
     if phase == "half_transform":
-        xsz = 8 if prec == "fp64" else 4
-        bsz = 5 if prec == "int8" else 8  # five int8 digit planes, or FP64
-        byts = naux * nbf * nbf * bsz + naux * nbf * nocc * xsz + nbf * nbf * 8 + nocc * nbf * xsz
-        flops = 2.0 * naux * nbf * nbf * nocc
         if prec == "fp64":
-            return dict(bound="mfma", achieved=flops / t / 1e12, peak=PEAK["fp64_TF"], unit="TFLOP/s",
-                        frac=flops / t / 1e12 / PEAK["fp64_TF"], traffic=None,
-                        kernel="k_half<double>", algorithmic_flops=flops)
-        return dict(bound="hbm", achieved=byts / t / 1e9, peak=PEAK["hbm_GBps"], unit="GB/s",
-                    frac=byts / t / 1e9 / PEAK["hbm_GBps"], traffic=None,
-                    kernel={"fp32": "k_half<float>", "int8": "k_half_i8", "bf16x6": "k_half_bf16<3>",
-                            "bf16x3": "k_half_bf16<2>"}[prec], algorithmic_bytes=byts)
+            return mfma(2.0 * naux * nbf * nbf * nocc, PEAK["fp64_TF"])
+        if prec == "int8":  # 4 B digit planes in, 4 X digit planes out
+            return hbm(4.0 * naux * nbf * nbf + 4.0 * naux * nbf * nocc)
+        return hbm(8.0 * naux * nbf * nbf + 4.0 * naux * nbf * nocc)  # FP64 B in, FP32 X out
     if phase == "exchange":
-        flops = float(naux * nocc * nbf * (nbf + 1))
-        pk = {"fp64": PEAK["fp64_TF"], "fp32": PEAK["fp32_TF"], "int8": PEAK["fp32_TF"],
-              "bf16x6": PEAK["bf16_TF"] / 6, "bf16x3": PEAK["bf16_TF"] / 3}[prec]
-        return dict(bound="mfma", achieved=flops / t / 1e12, peak=pk, unit="TFLOP/s", frac=flops / t / 1e12 / pk,
-                    traffic=None, kernel="k_syrk", algorithmic_flops=flops)
-    if phase == "coulomb":
-        byts = naux * nbf * (nbf + 1) / 2 * 8
-        return dict(bound="hbm", achieved=byts / t / 1e9, peak=PEAK["hbm_GBps"], unit="GB/s",
-                    frac=byts / t / 1e9 / PEAK["hbm_GBps"], traffic=None, kernel="k_jpart<double>",
-                    algorithmic_bytes=byts)
+        f = float(naux * nocc * nbf * (nbf + 1))
+        if prec == "int8":
+            return mfma(13 * f, PEAK["int8_TOPS"], "TOP/s")
+        pk = {"fp64": PEAK["fp64_TF"], "fp32": PEAK["fp32_TF"], "bf16x6": PEAK["bf16_TF"] / 6,
+              "bf16x3": PEAK["bf16_TF"] / 3}[prec]
+        return mfma(f, pk)
+    if phase == "gamma":
+        if prec == "int8":  # five digit planes of the lower triangle
+            return hbm(5.0 * naux * npair)
+        return None  # partials ride on the half-transform
+    if phase == "coulomb_j":
+        return hbm(8.0 * naux * npair)
     return None
+
+
+def perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3):
+    """J/K of D_k = D + eps sym(R_k), R_k ~ N(0,1) from one seeded generator.
+    D_k is indefinite, so K comes from its signed eigen-factor in one batched
+    build, K = K(X+) - K(X-); J from D_k. Max |dJ|, |dK| and |dE| of each mode
+    against the FP64 build of the same D_k."""
+    rng = np.random.default_rng(seed)
+    n = D.shape[0]
+    out = {p: dict(max_abs_dJ=0.0, max_abs_dK=0.0, max_abs_dE=0.0) for p in modes if p != "fp64"}
+    for _ in range(count):
+        R = rng.standard_normal((n, n))
+        Dk = D + eps * 0.5 * (R + R.T)
+        s, X = np.linalg.eigh(Dk)
+        Xp = np.ascontiguousarray((X[:, s > 0] * np.sqrt(s[s > 0])).T)
+        Xm = np.ascontiguousarray((X[:, s < 0] * np.sqrt(-s[s < 0])).T)
+        res = {}
+        for p in modes:
+            (J, Kp), (_, Km) = ctx.build_batch([(Dk, Xp, Xp.shape[0]), (None, Xm, Xm.shape[0])], prec=p)
+            res[p] = (J, Kp - Km)
+        J0, K0 = res["fp64"]
+        E0 = energy_terms(Dk, J0, K0)
+        for p in out:
+            J, K = res[p]
+            o = out[p]
+            o["max_abs_dJ"] = max(o["max_abs_dJ"], float(np.abs(J - J0).max()))
+            o["max_abs_dK"] = max(o["max_abs_dK"], float(np.abs(K - K0).max()))
+            o["max_abs_dE"] = max(o["max_abs_dE"], abs(energy_terms(Dk, J, K) - E0))
+    return out
 
 
 def cpu_baseline(sym, xyz_ang, threads, nbuilds=3, label="C20H42"):
@@ -183,6 +235,8 @@ def main():
         acc[p] = dict(dE=energy_terms(D, J, K) - ref[2], max_abs_dJ=float(np.abs(J - ref[0]).max()),
                       max_abs_dK=float(np.abs(K - ref[1]).max()))
     log(f"[bench] accuracy vs FP64: {json.dumps(acc)}")
+    pert = perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3)
+    log(f"[bench] perturbed densities (10, eps 1e-3): {json.dumps(pert)}")
 
     runs = {}
     for p in modes:
@@ -196,10 +250,10 @@ def main():
     rate, ms, phases = runs[prec]
     rate64 = runs["fp64"][0]
 
-    dom = max(("half_transform", "coulomb", "exchange"), key=lambda k: phases[k])
+    timed = ("half_transform", "gamma", "coulomb_j", "exchange")
+    dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec)), key=lambda k: phases[k])
     roof = roofline(dom, phases[dom], nbf, naux, nocc, prec)
-    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p) for k in ("half_transform", "coulomb", "exchange")}
-            for p in modes}
+    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p) for k in timed} for p in modes}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -241,6 +295,7 @@ def main():
                           "kernels": {k: (None if v is None else {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                                   for kk, vv in v.items()})
                                       for k, v in kern[p].items()}} for p in modes},
+            "perturbed_densities": pert,
             "E_rhf_fp64": res.E,
             "setup_s": {"host_integrals": round(t_ints, 1), "B_device": round(t_B, 2), "scf": round(t_scf, 1)},
         },

@@ -118,11 +118,12 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, 
 
 /* Per-phase timing with HIP events on the build's stream (no counterpart in
  * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When
- * enabled, each build (or batch) records events around its five phases:
+ * enabled, each build (or batch) records events around its six phases:
  * staging of D and C, half-transform (with the fused gamma partials of a
- * single FP64/FP32/bf16 build), Coulomb (gamma and J partials), exchange (K
- * products and K output) and the J output. read_timing returns the summed milliseconds per phase (ms[5])
- * and the number of builds recorded since the last read, then resets. */
+ * single FP64/FP32/bf16 build), gamma, J partials, exchange (K products and
+ * K output) and the J output. read_timing returns the summed milliseconds per
+ * phase (ms[6]) and the number of builds recorded since the last read, then
+ * resets. */
 int nopt_jk_set_timing(nopt_jk_ctx* ctx, int enable);
 int nopt_jk_read_timing(nopt_jk_ctx* ctx, double* ms, long* builds);
 /* The context's own stream (hipStream_t). */

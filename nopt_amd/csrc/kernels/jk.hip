@@ -584,7 +584,7 @@ struct nopt_jk_ctx {
 };
 
 namespace {
-constexpr int kPhases = 5;  // copy-in, half-transform, Coulomb, exchange, finalize
+constexpr int kPhases = 6;  // staging, half-transform, gamma, J partials, exchange, J output
 constexpr int kTimedBuilds = 512;
 }  // namespace
 
@@ -723,8 +723,8 @@ void stage_rows(const double* src, long lds, int rows, int cols, T* dst, long ld
 // are stacked into one padded factor, so the half-transform reads B (or its
 // digit planes) once for all of them; gamma and J are formed for every
 // density in one pass over B each; K is one triangular product per density.
-// Phases (for the optional timing): staging, half-transform, Coulomb (gamma +
-// J partials), exchange (K products and K output), J output.
+// Phases (for the optional timing): staging, half-transform, gamma, J
+// partials, exchange (K products and K output), J output.
 int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStream_t s) {
   const int nbf = (int)c->nbf, naux = (int)c->naux;
   if (prec < 0 || prec > 4) return set_error(NOPT_EINVAL, "unknown precision mode");
@@ -886,6 +886,10 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       switch (nj) { NOPT_GF(1) NOPT_GF(2) NOPT_GF(3) NOPT_GF(4) }
 #undef NOPT_GF
     }
+    HIPCHK(hipGetLastError());
+  }
+  mark(3);
+  if (nj) {
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                  \
   case ND:                                                                                           \
@@ -895,7 +899,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
 #undef NOPT_JP
     HIPCHK(hipGetLastError());
   }
-  mark(3);
+  mark(4);
   for (int d = 0; d < nd; ++d) {
     if (!it[d].K) continue;
     const int kpad = (int)(off[d + 1] - off[d]);
@@ -927,12 +931,12 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     HIPCHK(hipGetLastError());
   }
-  mark(4);
+  mark(5);
   for (int k = 0; k < nj; ++k)
     k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(Jpart + k * nsj * nn, nsj, nullptr, 0, 1, KT, nbf,
                                                             it[jl[k]].J, nullptr);
   HIPCHK(hipGetLastError());
-  mark(5);
+  mark(6);
   if (tm) ++c->timed;
   return NOPT_OK;
 }

@@ -142,11 +142,11 @@ class JKContext:
     def set_timing(self, enable=True):
         L.check(L.lib().nopt_jk_set_timing(self._h, 1 if enable else 0))
 
-    PHASES = ("copy_in", "half_transform", "coulomb", "exchange", "finalize")
+    PHASES = ("staging", "half_transform", "gamma", "coulomb_j", "exchange", "j_output")
 
     def read_timing(self):
         """{phase: summed ms}, builds recorded since the last read."""
-        ms = np.zeros(5)
+        ms = np.zeros(len(self.PHASES))
         nb = ctypes.c_long()
         L.check(L.lib().nopt_jk_read_timing(self._h, L.dptr(ms), ctypes.byref(nb)))
         return dict(zip(self.PHASES, ms.tolist())), nb.value
