@@ -129,6 +129,19 @@ int nopt_jk_read_timing(nopt_jk_ctx* ctx, double* ms, long* builds);
 /* The context's own stream (hipStream_t). */
 void* nopt_jk_stream(nopt_jk_ctx* ctx);
 
+/* HIP graph of device builds (no counterpart in NOPT): between begin and end,
+ * nopt_jk_build_device / nopt_jk_build_batch_device calls on `stream` (of
+ * one or several contexts on the device, e.g. a ragged batch of molecules)
+ * are recorded instead of run; launch replays them all with one call. The
+ * inputs and outputs are the device buffers named at capture time (update
+ * their contents between launches). Run one build of each shape before
+ * capturing: buffers cannot grow during a capture (NOPT_ESTATE). */
+typedef struct nopt_graph nopt_graph;
+int nopt_graph_capture_begin(int device, void* stream);
+int nopt_graph_capture_end(int device, void* stream, nopt_graph** out);
+int nopt_graph_launch(nopt_graph* graph, void* stream);
+void nopt_graph_destroy(nopt_graph* graph);
+
 /* Device-memory plumbing for callers without a HIP runtime binding.
  * kind: 0 host->device, 1 device->host, 2 device->device. */
 void* nopt_dev_malloc(int device, size_t bytes);

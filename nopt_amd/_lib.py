@@ -69,6 +69,10 @@ GPU_SYMBOLS = [
     ("nopt_dev_free", None, [_vp]),
     ("nopt_dev_memcpy", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int]),
     ("nopt_dev_synchronize", ctypes.c_int, [ctypes.c_int]),
+    ("nopt_graph_capture_begin", ctypes.c_int, [ctypes.c_int, _vp]),
+    ("nopt_graph_capture_end", ctypes.c_int, [ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    ("nopt_graph_launch", ctypes.c_int, [_vp, _vp]),
+    ("nopt_graph_destroy", None, [_vp]),
     # NOPT-named RI entry points (include/nopt_ri.h)
     ("RI_gpu_attach", ctypes.c_int, [_c_double_p, _c_double_p, ctypes.c_long, ctypes.c_long, ctypes.c_int]),
     ("RI_gpu_attach_molecule", ctypes.c_int, [ctypes.c_int, _c_int_p, _c_double_p, ctypes.c_char_p,

@@ -540,6 +540,10 @@ struct nopt_jk_ctx {
   long ld64 = 0;
   int mtiles = 0, ktn = 0, ntiles = 0;
   hipStream_t stream = nullptr;
+  // second stream for the bandwidth-bound Coulomb pair, which runs beside the
+  // MFMA-bound exchange; fork/join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* B64 = nullptr;
   // per-build work buffers (grown on demand)
   long cap_ldx = 0;
@@ -600,6 +604,13 @@ const char* hip_msg(hipError_t e) { return hipGetErrorString(e); }
                        std::string(#call) + ": " + hip_msg(_e));                          \
   } while (0)
 
+// returns a HIP error from a helper that itself returns hipError_t
+#define CHK_(call)                    \
+  do {                                \
+    hipError_t e_ = (call);           \
+    if (e_ != hipSuccess) return e_;  \
+  } while (0)
+
 int dev_alloc(void** p, size_t bytes, const char* what) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
@@ -618,8 +629,16 @@ void dev_free(P*& p) {
 }
 
 // Grows a device buffer to at least `bytes`.
+// set between nopt_graph_capture_begin and _end on this thread: device buffers
+// may not grow while a stream is being captured
+thread_local bool t_capturing = false;
+
 int ensure(void** p, size_t* cap, size_t bytes, const char* what) {
   if (*cap >= bytes && *p) return NOPT_OK;
+  if (t_capturing)
+    return set_error(NOPT_ESTATE, std::string("the ") + what +
+                                      " buffer would grow during graph capture; run one build of the same shape "
+                                      "before capturing");
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
@@ -812,6 +831,48 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   auto mark = [&](int ph) {
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
   };
+  // Coulomb pair (gamma, J partials) and the J output. Without timing they
+  // run on the side stream, forked after the staging (or after the
+  // half-transform when gamma rides on it) and joined before the build ends,
+  // so that they overlap the exchange.
+  const bool overlap = !tm && nj > 0 && anyK;
+  hipStream_t sj = overlap ? c->side : s;
+  auto cmark = [&](int ph) {
+    if (!overlap) mark(ph);
+  };
+  auto coulomb = [&]() -> hipError_t {
+    if (fused) {
+      k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
+    } else if (i8) {
+      CHK_(nopt::launch_gamma_i8(nj, naux, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, c->Dp, ldb, c->dstride,
+                                 c->gamma, naux));
+    } else {
+#define NOPT_GF(ND)                                                                                         \
+  case ND:                                                                                                  \
+    k_gamma_f64<ND><<<naux, 256, 0, sj>>>(c->B64, c->ld64, nbf, c->Dp, ldb, c->dstride, c->gamma, naux); \
+    break;
+      switch (nj) { NOPT_GF(1) NOPT_GF(2) NOPT_GF(3) NOPT_GF(4) }
+#undef NOPT_GF
+    }
+    CHK_(hipGetLastError());
+    cmark(3);
+    dim3 gj((nbf + 1) / 2, nsj);
+#define NOPT_JP(ND)                                                                                     \
+  case ND:                                                                                              \
+    k_jpart<ND><<<gj, 256, 0, sj>>>(c->B64, c->ld64, nbf, naux, c->gamma, naux, Jpart, (long)nsj * nn); \
+    break;
+    switch (nj) { NOPT_JP(1) NOPT_JP(2) NOPT_JP(3) NOPT_JP(4) }
+#undef NOPT_JP
+    return hipGetLastError();
+  };
+  auto coulomb_out = [&]() -> hipError_t {
+    for (int k = 0; k < nj; ++k)
+      k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, sj>>>(Jpart + k * nsj * nn, nsj, nullptr, 0, 1, KT, nbf,
+                                                               it[jl[k]].J, nullptr);
+    return hipGetLastError();
+  };
+  // 0: no J; 1: forked after staging; 2: in line (after the half-transform)
+  int coulomb_at = nj ? 2 : 0;
   mark(0);
   // densities of the J list into slots 0..nj-1 (ld = ldb); orbital blocks
   // into the stacked factor at column offsets off[d] (zero rows between)
@@ -839,6 +900,13 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       stage_rows<float>(c->Dp, ldb, 0, 0, (float*)c->Cp + ncol * ldb, ldb, tail, s);
   }
   HIPCHK(hipGetLastError());
+  if (overlap && !fused) {
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    HIPCHK(coulomb());
+    HIPCHK(coulomb_out());
+    coulomb_at = 1;
+  }
   mark(1);
   if (anyK) {
     dim3 gh(mt, naux, ngroups);
@@ -872,33 +940,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     HIPCHK(e);
   }
   mark(2);
-  if (nj) {
-    if (fused) {
-      k_gamma<<<(naux + 255) / 256, 256, 0, s>>>(c->gpart, c->mtiles, naux, c->gamma);
-    } else if (i8) {
-      HIPCHK(nopt::launch_gamma_i8(nj, naux, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, c->Dp, ldb, c->dstride,
-                                   c->gamma, naux));
-    } else {
-#define NOPT_GF(ND)                                                                                        \
-  case ND:                                                                                                 \
-    k_gamma_f64<ND><<<naux, 256, 0, s>>>(c->B64, c->ld64, nbf, c->Dp, ldb, c->dstride, c->gamma, naux); \
-    break;
-      switch (nj) { NOPT_GF(1) NOPT_GF(2) NOPT_GF(3) NOPT_GF(4) }
-#undef NOPT_GF
-    }
-    HIPCHK(hipGetLastError());
+  if (overlap && fused) {
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    HIPCHK(coulomb());
+    HIPCHK(coulomb_out());
+    coulomb_at = 1;
   }
-  mark(3);
-  if (nj) {
-    dim3 gj((nbf + 1) / 2, nsj);
-#define NOPT_JP(ND)                                                                                  \
-  case ND:                                                                                           \
-    k_jpart<ND><<<gj, 256, 0, s>>>(c->B64, c->ld64, nbf, naux, c->gamma, naux, Jpart, (long)nsj * nn); \
-    break;
-    switch (nj) { NOPT_JP(1) NOPT_JP(2) NOPT_JP(3) NOPT_JP(4) }
-#undef NOPT_JP
-    HIPCHK(hipGetLastError());
-  }
+  if (coulomb_at == 2) HIPCHK(coulomb());
   mark(4);
   for (int d = 0; d < nd; ++d) {
     if (!it[d].K) continue;
@@ -932,10 +981,11 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     HIPCHK(hipGetLastError());
   }
   mark(5);
-  for (int k = 0; k < nj; ++k)
-    k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(Jpart + k * nsj * nn, nsj, nullptr, 0, 1, KT, nbf,
-                                                            it[jl[k]].J, nullptr);
-  HIPCHK(hipGetLastError());
+  if (coulomb_at == 2) HIPCHK(coulomb_out());
+  if (overlap) {
+    HIPCHK(hipEventRecord(c->ev_join, c->side));
+    HIPCHK(hipStreamWaitEvent(s, c->ev_join, 0));
+  }
   mark(6);
   if (tm) ++c->timed;
   return NOPT_OK;
@@ -1012,6 +1062,9 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   int rc = NOPT_OK;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return set_error(NOPT_EHIP, std::string("nopt_jk_create: ") + hip_msg(e));
@@ -1077,6 +1130,10 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
     dev_free(c->xK[i]);
   }
   for (auto& ev : c->tev) (void)hipEventDestroy(ev);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1116,6 +1173,65 @@ int nopt_jk_read_timing(nopt_jk_ctx* c, double* ms, long* builds) {
 }
 
 void* nopt_jk_stream(nopt_jk_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+struct nopt_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int nopt_graph_capture_begin(int device, void* stream) {
+  nopt::clear_error();
+  if (!stream) return set_error(NOPT_EINVAL, "nopt_graph_capture_begin: a non-default stream is required");
+  if (t_capturing) return set_error(NOPT_ESTATE, "nopt_graph_capture_begin: already capturing on this thread");
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  t_capturing = true;
+  return NOPT_OK;
+}
+
+int nopt_graph_capture_end(int device, void* stream, nopt_graph** out) {
+  nopt::clear_error();
+  if (!stream || !out) return set_error(NOPT_EINVAL, "nopt_graph_capture_end: null stream or output");
+  *out = nullptr;
+  if (!t_capturing) return set_error(NOPT_ESTATE, "nopt_graph_capture_end: no capture in progress");
+  t_capturing = false;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return set_error(NOPT_EHIP, std::string("hipGraphInstantiate: ") + hip_msg(e));
+  }
+  auto* gr = new (std::nothrow) nopt_graph;
+  if (!gr) {
+    (void)hipGraphExecDestroy(ex);
+    (void)hipGraphDestroy(g);
+    return set_error(NOPT_ENOMEM, "nopt_graph_capture_end: host allocation failed");
+  }
+  gr->device = device;
+  gr->graph = g;
+  gr->exec = ex;
+  *out = gr;
+  return NOPT_OK;
+}
+
+int nopt_graph_launch(nopt_graph* g, void* stream) {
+  nopt::clear_error();
+  if (!g) return set_error(NOPT_EINVAL, "nopt_graph_launch: null graph");
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return NOPT_OK;
+}
+
+void nopt_graph_destroy(nopt_graph* g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
 
 void* nopt_dev_malloc(int device, size_t bytes) {
   void* p = nullptr;

@@ -290,19 +290,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // Dp + d*dstride), over the lower triangle (off-diagonal weight 2) with B
 // rebuilt exactly from its five digit planes:
 //   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
-// One block per auxiliary function; each wave walks row pairs (p, nbf-1-p)
-// in 8-column chunks, so B is read once for all ND densities.
-template <int ND>
+// One block per QB consecutive auxiliary functions (so that each density
+// chunk is read from cache once per QB functions); each wave walks row pairs
+// (p, nbf-1-p) in 8-column chunks.
+template <int ND, int QB>
 __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd, long bplane,
-                                                  const int* __restrict__ Be, long ld8, int nbf,
+                                                  const int* __restrict__ Be, long ld8, int nbf, int naux,
                                                   const double* __restrict__ Dp, long ldd, long dstride,
                                                   double* __restrict__ gamma, long gstride) {
   __shared__ double red[4];
-  const long Q = blockIdx.x;
+  const int qb0 = blockIdx.x * QB;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double acc[ND];
+  double acc[QB][ND];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+  for (int q = 0; q < QB; ++q)
+#pragma unroll
+    for (int d = 0; d < ND; ++d) acc[q][d] = 0.0;
   const int npairs = (nbf + 1) / 2;
   for (int p = w; p < npairs; p += 4) {
     const int m1 = p, m2 = nbf - 1 - p;
@@ -310,36 +313,49 @@ __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd,
     for (int ch = lane; ch < c1 + c2; ch += 64) {
       const int m = ch < c1 ? m1 : m2;
       const int n0 = 8 * (ch < c1 ? ch : ch - c1);
-      const long row = Q * nbf + m;
-      const int8_t* b = Bd + row * ld8 + n0;
-      int8_t dg[5][8];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
-      double bv[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int n = n0 + j;
-        const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
-        const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
-        const double wgt = n < m ? 2.0 : (n == m ? 1.0 : 0.0);
-        bv[j] = wgt * (0x1p-7 * (double)hi + 0x1p-28 * (double)lo);
-      }
-      const int sc = Be[row] - 6;
+      double dv[ND][8];
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
         const double* dr = Dp + d * dstride + (long)m * ldd + n0;
-        double s = 0.0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) s += bv[j] * dr[j];
-        acc[d] += ldexp(s, sc);
+        for (int j = 0; j < 8; ++j) {
+          const int n = n0 + j;
+          dv[d][j] = (n < m ? 2.0 : (n == m ? 1.0 : 0.0)) * dr[j];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        if (qb0 + q >= naux) break;
+        const long row = (long)(qb0 + q) * nbf + m;
+        const int8_t* b = Bd + row * ld8 + n0;
+        int8_t dg[5][8];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
+        double bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
+          const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
+          bv[j] = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
+        }
+        const int sc = Be[row] - 6;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          double sacc = 0.0;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) sacc += bv[j] * dv[d][j];
+          acc[q][d] += ldexp(sacc, sc);
+        }
       }
     }
   }
 #pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    const double tot = block_sum256(acc[d], red);
-    if (threadIdx.x == 0) gamma[d * gstride + Q] = tot;
-  }
+  for (int q = 0; q < QB; ++q)
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const double tot = block_sum256(acc[q][d], red);
+      if (threadIdx.x == 0 && qb0 + q < naux) gamma[d * gstride + qb0 + q] = tot;
+    }
 }
 
 // ------------------------------------------------------------------ exchange
@@ -554,11 +570,13 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
 
 hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                            int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride) {
+  constexpr int QB = 4;
+  const unsigned nb = (unsigned)((naux + QB - 1) / QB);
   switch (nd) {
-    case 1: k_gamma_i8<1><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
-    case 2: k_gamma_i8<2><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
-    case 3: k_gamma_i8<3><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
-    case 4: k_gamma_i8<4><<<naux, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Dp, ldd, dstride, gamma, gstride); break;
+    case 1: k_gamma_i8<1, QB><<<nb, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
+    case 2: k_gamma_i8<2, QB><<<nb, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
+    case 3: k_gamma_i8<3, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
+    case 4: k_gamma_i8<4, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -139,6 +139,16 @@ class JKContext:
         p = L.PREC[prec] if isinstance(prec, str) else int(prec)
         L.check(L.lib().nopt_jk_build_device(self._h, dD, dC, int(nocc), dJ, dK, p, stream or None))
 
+    def build_batch_device(self, items, prec="fp64", stream=0):
+        """Device-pointer batch: items of (dD, dC, nocc, dJ, dK) integer
+        addresses (0 for none), enqueued on `stream`."""
+        nd = len(items)
+        p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+        col = lambda i: (ctypes.c_void_p * nd)(*[(it[i] or None) for it in items])
+        L.check(L.lib().nopt_jk_build_batch_device(self._h, nd, col(0), col(1),
+                                                   (ctypes.c_long * nd)(*[int(it[2]) for it in items]),
+                                                   col(3), col(4), p, stream or None))
+
     def set_timing(self, enable=True):
         L.check(L.lib().nopt_jk_set_timing(self._h, 1 if enable else 0))
 
@@ -199,6 +209,13 @@ class DeviceArray:
         L.check(L.lib().nopt_dev_memcpy(d.ptr, a.ctypes.data, d.nbytes, 0))
         return d
 
+    def copy_from(self, a):
+        """Overwrites the buffer with host array `a` (same size)."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        L.check(L.lib().nopt_dev_memcpy(self.ptr, a.ctypes.data, self.nbytes, 0))
+
     def to_host(self):
         out = np.empty(self.shape)
         L.check(L.lib().nopt_dev_memcpy(out.ctypes.data, self.ptr, self.nbytes, 1))
@@ -211,6 +228,53 @@ class DeviceArray:
 
     def __del__(self):
         self.free()
+
+
+class Graph:
+    """Builds captured into one HIP graph (include/nopt_jk.h, nopt_graph_*).
+
+        with Graph.capture(stream) as g:      # stream: a context's stream
+            ctx1.build_device(..., stream=stream)
+            ctx2.build_device(..., stream=stream)
+        g.launch(stream)                       # replays every captured build
+    """
+
+    def __init__(self, handle, device):
+        self._h, self.device = handle, device
+
+    @classmethod
+    def capture(cls, stream, device=0):
+        return _Capture(stream, device)
+
+    def launch(self, stream):
+        L.check(L.lib().nopt_graph_launch(self._h, stream))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().nopt_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class _Capture:
+    def __init__(self, stream, device):
+        self.stream, self.device, self.graph = stream, device, None
+
+    def __enter__(self):
+        L.check(L.lib().nopt_graph_capture_begin(self.device, self.stream))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        rc = L.lib().nopt_graph_capture_end(self.device, self.stream, ctypes.byref(h))
+        if exc_type is None:
+            L.check(rc)
+            self.graph = Graph(h, self.device)
+        elif rc == 0 and h.value:
+            L.lib().nopt_graph_destroy(h)
+        return False
 
 
 def synchronize(device=0):
