@@ -43,11 +43,13 @@ def energy_terms(D, J, K):
 
 
 def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, timing=False):
+    """Builds/s of `steps` builds after `warmup` (device synchronised on both
+    sides). With timing, a second pass of `steps` builds records HIP events
+    around each phase; phases then run one after the other on one stream,
+    while the timed pass lets the Coulomb pair overlap the exchange."""
     for _ in range(warmup):
         ctx.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
     synchronize()
-    if timing:
-        ctx.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         ctx.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
@@ -55,6 +57,10 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
     dt = time.perf_counter() - t0
     phases = None
     if timing:
+        ctx.set_timing(True)
+        for _ in range(steps):
+            ctx.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
+        synchronize()
         ph, nb = ctx.read_timing()
         ctx.set_timing(False)
         phases = {k: v / max(nb, 1) for k, v in ph.items()}
@@ -184,12 +190,28 @@ def main():
                     help="headline mode; auto = fastest mode with |dE| < 1 uEh against FP64")
     ap.add_argument("--alkane", type=int, default=20, help="carbon count of the all-trans alkane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,int8,fp32,bf16x6,bf16x3",
+                    help="comma-separated modes to time (fp64 is always built as the reference)")
+    ap.add_argument("--no-perturbed", action="store_true", help="skip the perturbed-density accuracy check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--cpu-threads", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (aux-index sharded) design is "
               "described in DESIGN.md and is not launched.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        from nopt_amd.jk import DeviceArray, synchronize
+        n = args.copy_probe * (1 << 20) // 8
+        a, b = DeviceArray((n,)), DeviceArray((n,))
+        from nopt_amd import _lib as L
+        for _ in range(5):
+            L.check(L.lib().nopt_dev_memcpy(b.ptr, a.ptr, a.nbytes, 2))
+        synchronize()
+        print(json.dumps({"copy_probe_bytes": a.nbytes, "copies": 5}))
+        return
 
     from nopt_amd import geometry as G
     from nopt_amd.molecule import Molecule
@@ -223,7 +245,7 @@ def main():
     dJ, dK = DeviceArray((nbf, nbf)), DeviceArray((nbf, nbf))
 
     # accuracy of each mode against the FP64 build on the same density
-    modes = ("fp64", "int8", "fp32", "bf16x6", "bf16x3")
+    modes = tuple(["fp64"] + [p for p in args.modes.split(",") if p and p != "fp64"])
     acc = {}
     ref = None
     for p in modes:
@@ -235,7 +257,7 @@ def main():
         acc[p] = dict(dE=energy_terms(D, J, K) - ref[2], max_abs_dJ=float(np.abs(J - ref[0]).max()),
                       max_abs_dK=float(np.abs(K - ref[1]).max()))
     log(f"[bench] accuracy vs FP64: {json.dumps(acc)}")
-    pert = perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3)
+    pert = None if args.no_perturbed else perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3)
     log(f"[bench] perturbed densities (10, eps 1e-3): {json.dumps(pert)}")
 
     runs = {}

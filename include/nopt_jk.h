@@ -112,9 +112,10 @@ int nopt_jk_fock_energy(nopt_jk_ctx* ctx, const double* H, const double* D, cons
  * D = C^T C (first nocc rows), F = H + 2J - K, E = sum D.*(H+F), and
  * G = dE/dC = 4 C F (nocc x nbf, rows are orbitals). This is the adjoint of
  * E_RHF(C); its occupied-virtual projection is NOPT's orbital gradient
- * g_ia = 4 F_ai (src/converger_2_1.cpp:54-62). G may be NULL. */
+ * g_ia = 4 F_ai (src/converger_2_1.cpp:54-62). G and F (nbf x nbf) may be
+ * NULL. */
 int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, long nocc,
-                            int prec, double* E, double* G);
+                            int prec, double* E, double* G, double* F);
 
 /* Per-phase timing with HIP events on the build's stream (no counterpart in
  * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When

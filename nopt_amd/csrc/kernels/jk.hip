@@ -1460,7 +1460,7 @@ int nopt_jk_fock_energy(nopt_jk_ctx* c, const double* H, const double* D, const 
 }
 
 int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, long nocc, int prec,
-                            double* E, double* G) {
+                            double* E, double* G, double* F) {
   nopt::clear_error();
   int rc = check_ctx(c, true);
   if (rc) return rc;
@@ -1488,6 +1488,7 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, lo
                           c->stream));
     HIPCHK(hipMemcpyAsync(G, c->dG, sizeof(double) * nocc * nbf, hipMemcpyDeviceToHost, c->stream));
   }
+  if (F) HIPCHK(hipMemcpyAsync(F, c->dF, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
   double e = 0.0;
   HIPCHK(hipMemcpyAsync(&e, c->dE, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

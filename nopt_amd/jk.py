@@ -177,18 +177,20 @@ class JKContext:
                                             float(cj), float(ck), L.dptr(F), ctypes.byref(E)))
         return F, E.value
 
-    def energy_gradient(self, H, C, nocc=None, prec="fp64"):
-        """E = sum D(H+F) (no V_nuc) and G = dE/dC = 4 C F (nocc x nbf)."""
+    def energy_gradient(self, H, C, nocc=None, prec="fp64", fock=False):
+        """E = sum D(H+F) (no V_nuc) and G = dE/dC = 4 C F (nocc x nbf);
+        with fock=True also F."""
         n = self.nbf
         Hm = self._arr(H, (n, n))
         Cm = np.ascontiguousarray(C, dtype=np.float64)
         nocc = int(nocc or Cm.shape[0])
         G = np.empty((nocc, n))
+        F = np.empty((n, n)) if fock else None
         E = ctypes.c_double()
         p = L.PREC[prec] if isinstance(prec, str) else int(prec)
         L.check(L.lib().nopt_jk_energy_gradient(self._h, L.dptr(Hm), L.dptr(Cm), nocc, p,
-                                                ctypes.byref(E), L.dptr(G)))
-        return E.value, G
+                                                ctypes.byref(E), L.dptr(G), L.dptr(F) if fock else None))
+        return (E.value, G, F) if fock else (E.value, G)
 
 
 class DeviceArray:

@@ -102,64 +102,71 @@ def rotate(C, nocc, k):
 
 
 def minimize_orbitals(mol, ctx, H, C0, prec="fp64", g_conv=1e-6, e_conv=1e-11, max_it=300, memory=8,
-                      max_step=0.3, verbose=False):
+                      trust=0.5, verbose=False):
     """Direct minimisation of E_RHF over orbital rotations C <- C exp(kappa)
     with the GPU energy/adjoint (dE/dC_occ = 4 F C_occ), in the way of NOPT's
     SOSCF/L-BFGS engine (src/converger_2_1.cpp:22-123, src/l-bfgs_2_1.cpp:33-125):
-    gradient g_ai = 4 F_ai in the current orbital basis, a diagonal
-    preconditioner 4 (e_a - e_i) from the starting Fock matrix, L-BFGS steps
-    capped at `max_step`, and the orbitals re-based after every step. C0 must be
-    orthonormal in the metric S (C0^T S C0 = 1). Returns SCFResult(E, C, iters,
-    gmax, history)."""
-    nocc, n = mol.nocc, C0.shape[1]
+    gradient g_ai = 4 F_ai in the current orbitals, the diagonal Hessian
+    estimate 4 (F_aa - F_ii) as preconditioner, L-BFGS directions within a
+    trust radius on max|kappa|, the orbitals re-based after every step, and a
+    step that raises the energy rejected (radius halved, history cleared).
+    C0 must be orthonormal in the metric S. Returns SCFResult(E, C, iters,
+    gmax, history, converged)."""
+    nocc = mol.nocc
     C = np.array(C0, dtype=np.float64)
-    # preconditioner from the Fock matrix of the starting orbitals
-    Co = C[:, :nocc]
-    D = Co @ Co.T
-    J, K = ctx.build(D, Co.T, nocc, prec=prec)
-    F, _ = ctx.fock_energy(H, D, J, K, 2.0, 1.0)
-    eps = np.diag(C.T @ F @ C)
-    hdiag = np.maximum(4.0 * (eps[nocc:, None] - eps[None, :nocc]), 0.2)
-    S_hist, Y_hist = [], []
-    g_prev = step_prev = None
+
+    def evaluate(Cc):
+        Co = np.ascontiguousarray(Cc[:, :nocc].T)
+        E_el, _, F = ctx.energy_gradient(H, Co, nocc, prec=prec, fock=True)
+        Fmo = Cc.T @ F @ Cc
+        return E_el + mol.e_nuc, 4.0 * Fmo[nocc:, :nocc], np.diag(Fmo)
+
+    E, g, eps = evaluate(C)
+    S_hist, Y_hist, hist = [], [], []
     E_prev = None
-    hist = []
     for it in range(max_it):
-        Co = np.ascontiguousarray(C[:, :nocc].T)
-        E_el, G = ctx.energy_gradient(H, Co, nocc, prec=prec)
-        E = E_el + mol.e_nuc
-        g = C[:, nocc:].T @ G.T  # (nvirt x nocc) = C_v^T (4 F C_o)
         gmax = float(np.abs(g).max())
-        hist.append((it, E, gmax))
+        hist.append((it, E, gmax, trust))
         if verbose:
-            print(f"MIN {it} E={E:.12f} |g|max={gmax:.3e}")
-        if g_prev is not None:
-            y = g - g_prev
-            sy = float(np.sum(step_prev * y))
-            if sy > 1e-14:
-                S_hist.append(step_prev)
-                Y_hist.append(y)
-                if len(S_hist) > memory:
-                    S_hist.pop(0)
-                    Y_hist.pop(0)
+            print(f"MIN {it} E={E:.12f} |g|max={gmax:.3e} trust={trust:.3f}")
         if gmax < g_conv and E_prev is not None and abs(E - E_prev) < e_conv:
             return SCFResult(E=E, C=C, iters=it, gmax=gmax, history=hist, converged=True)
-        # L-BFGS two-loop recursion with the diagonal preconditioner
+        hdiag = np.maximum(4.0 * (eps[nocc:, None] - eps[None, :nocc]), 0.2)
+        # L-BFGS two-loop recursion, H0 = diag(1/hdiag)
         q = g.copy()
-        alph = []
+        saved = []
         for s_, y_ in zip(reversed(S_hist), reversed(Y_hist)):
-            rho = 1.0 / np.sum(y_ * s_)
-            a = rho * np.sum(s_ * q)
-            alph.append((a, rho, s_, y_))
+            rho = 1.0 / float(np.sum(y_ * s_))
+            a = rho * float(np.sum(s_ * q))
+            saved.append((a, rho, s_, y_))
             q -= a * y_
         r = q / hdiag
-        for a, rho, s_, y_ in reversed(alph):
-            b = rho * np.sum(y_ * r)
-            r += (a - b) * s_
+        for a, rho, s_, y_ in reversed(saved):
+            r += (a - rho * float(np.sum(y_ * r))) * s_
         step = -r
+        if float(np.sum(step * g)) >= 0.0:  # not a descent direction: preconditioned gradient
+            step = -g / hdiag
+            S_hist.clear()
+            Y_hist.clear()
         big = float(np.abs(step).max())
-        if big > max_step:
-            step *= max_step / big
-        C = rotate(C, nocc, step)
-        g_prev, step_prev, E_prev = g, step, E
-    return SCFResult(E=E, C=C, iters=max_it, gmax=gmax, history=hist, converged=False)
+        if big > trust:
+            step *= trust / big
+        Cn = rotate(C, nocc, step)
+        En, gn, epsn = evaluate(Cn)
+        if En > E + 1e-12 * max(1.0, abs(E)):
+            trust = max(0.5 * trust, 1e-4)  # reject
+            S_hist.clear()
+            Y_hist.clear()
+            continue
+        y = gn - g
+        if float(np.sum(step * y)) > 1e-14:
+            S_hist.append(step)
+            Y_hist.append(y)
+            if len(S_hist) > memory:
+                S_hist.pop(0)
+                Y_hist.pop(0)
+        if big <= trust:
+            trust = min(2.0 * trust, 1.0)
+        E_prev = E
+        C, E, g, eps = Cn, En, gn, epsn
+    return SCFResult(E=E, C=C, iters=max_it, gmax=float(np.abs(g).max()), history=hist, converged=False)

@@ -65,7 +65,9 @@ def test_cas_pair_matches_oracle_and_single_builds(c2h4, prec):
         Js, Ks = ctx.build(D, C, no, prec=prec)
         say("  nocc", no, "max|dJ|", np.abs(J - J0).max(), "max|dK|", np.abs(K - K0).max(),
             "vs single", np.abs(J - Js).max(), np.abs(K - Ks).max())
-        assert np.abs(J - J0).max() < TOL[prec] and np.abs(K - K0).max() < TOL[prec]
+        # int8: relative to the largest element (the total-density factor carries sqrt(2) C_core)
+        assert np.abs(J - J0).max() < TOL[prec] * max(1.0, np.abs(J0).max())
+        assert np.abs(K - K0).max() < TOL[prec] * max(1.0, np.abs(K0).max())
         assert np.abs(J - Js).max() < 1e-12 and np.abs(K - Ks).max() < 1e-12
 
 
