@@ -964,7 +964,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       if (!xfused)
         HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
                                    rows, rows * ldk, (int*)xe, s));
-      HIPCHK(nopt::launch_syrk_i8(dim3(c->ntiles8, c->nsk8), s, xd, rows * ldk, xe, ldk, nbf, naux, c->Kpart));
+      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, rows * ldk, xe, ldk, nbf, naux, c->Kpart));
       k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
                                      it[d].K);
     } else {
@@ -1037,25 +1037,24 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
   c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
   {
-    // integer exchange: tiles x Q-splits sized to whole rounds of resident
-    // blocks (two per CU), so that the last round is not left mostly idle
+    // integer exchange: Q-splits in multiples of 8 (one contiguous range
+    // per XCD), sized so that the blocks of an XCD fill whole rounds of its
+    // resident slots (two blocks per CU)
     const int kt8 = (int)((nbf + nopt::kSyrkI8Tile - 1) / nopt::kSyrkI8Tile);
     c->ntiles8 = kt8 * (kt8 + 1) / 2;
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
       ncu = 256;
-    const long slots = 2L * ncu;
-    int best = 1;
+    const long slots = 2L * ncu / 8;  // per XCD
+    int best = 8;
     double best_eff = 0.0;
-    for (int r = 1; r <= 8; ++r) {
-      const int nsp = (int)std::max(1L, std::min((long)naux, slots * r / c->ntiles8));
-      const long blocks = (long)nsp * c->ntiles8;
-      const double eff = (double)blocks / (double)(slots * ((blocks + slots - 1) / slots));
-      if (eff > best_eff + 1e-9) {
+    for (int nsp = 8; nsp <= 64 && nsp <= std::max(8L, naux); nsp += 8) {
+      const long per = (long)nsp * c->ntiles8 / 8;
+      const double eff = (double)per / (double)(slots * ((per + slots - 1) / slots));
+      if (eff > best_eff + 0.02) {
         best_eff = eff;
         best = nsp;
       }
-      if (eff >= 0.9) break;
     }
     c->nsk8 = best;
   }

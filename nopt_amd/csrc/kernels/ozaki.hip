@@ -85,22 +85,28 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 
 // X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=4} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
 // for k, l < 4 (13 digit pairs; the products of equal k+l share one exact
-// int32 accumulator), combined in FP64. Tile: 64 rows x 16*NT columns, each
-// of the four waves owns 16 rows and all columns; k-slabs of 64 bytes, LDS
-// double buffered (one barrier per slab) with the next slab prefetched into
-// registers while the current one is multiplied.
-template <int NT>
+// int32 accumulator), combined in FP64. Tile: 64 rows x 16*NT columns; the
+// four waves form a (4/WC) x WC grid, each with IT = 4/(4/WC) row tiles and
+// JT = NT/WC column tiles of 16, so that a wave reads IT + JT operand tiles
+// from LDS per 13*IT*JT products. k-slabs of 64 bytes, LDS double buffered
+// (one barrier per slab) with the next B slab prefetched into registers while
+// the current one is multiplied.
+template <int NT, int WC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half_i8(
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
   constexpr int BMI = 64, BN = 16 * NT, KS = 64;
+  constexpr int WR = 4 / WC, IT = 4 / WR, JT = NT / WC;
+  static_assert(NT % WC == 0, "column tiles must split evenly over the waves");
   constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per buffer
   constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
   constexpr int CPT = CCH / 256;                      // = NT
   static_assert(CCH % 256 == 0, "C slab must split evenly over the block");
   __shared__ __attribute__((aligned(16))) int8_t sm[2][AB + CB];
+  __shared__ double rmax[WC > 1 ? WC : 1][WC > 1 ? BMI : 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wr = w / WC, wc = w % WC;
   const int m0 = blockIdx.x * BMI;
   const long Q = blockIdx.y;
   const int cg = blockIdx.z;
@@ -150,18 +156,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       else *d_ = c_hi[u_ < CH ? 0 : u_ - CH];                                                            \
     });                                                                                                  \
   } while (0)
-  v4i_t acc[NT][5];
+  v4i_t acc[IT][JT][5];
 #pragma unroll
-  for (int j = 0; j < NT; ++j)
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
-    for (int c = 0; c < 5; ++c) acc[j][c] = v4i_t{0, 0, 0, 0};
+    for (int j = 0; j < JT; ++j)
+#pragma unroll
+      for (int c = 0; c < 5; ++c) acc[i][j][c] = v4i_t{0, 0, 0, 0};
   const int nks = (int)(ld8 / KS);
   NOPT_GLOAD_A(0);
   NOPT_GLOAD_C(0);
   NOPT_LSTORE(0);
   __syncthreads();
   const int lr = lane & 15, lc = lane >> 4;
-  const int r = w * 16 + lr;
   for (int ks = 0; ks < nks; ++ks) {
     const int cur = ks & 1;
     // the B slab (HBM) is fetched before the products, the small C slab
@@ -169,32 +176,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
     const int8_t* As = sm[cur];
     const int8_t* Cs = sm[cur] + AB;
-    uint4 av[4];
+    uint4 av[IT][4];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) av[p] = *reinterpret_cast<const uint4*>(As + (p * BMI + r) * KS + 16 * swz(r, lc));
+    for (int i = 0; i < IT; ++i) {
+      const int r = wr * 16 * IT + i * 16 + lr;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * BMI + r) * KS + 16 * swz(r, lc));
+    }
     // one digit plane l of the C operand at a time: pairs (k, l), k+l <= 4
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const int col = j * 16 + lr;
+    for (int j = 0; j < JT; ++j) {
+      const int col = wc * 16 * JT + j * 16 + lr;
       const int8_t* cp = Cs + col * KS + 16 * swz(col, lc);
-      v4i_t* A = acc[j];
       uint4 bv = *reinterpret_cast<const uint4*>(cp);
-      A[0] = mfma_i8(av[0], bv, A[0]);
-      A[1] = mfma_i8(av[1], bv, A[1]);
-      A[2] = mfma_i8(av[2], bv, A[2]);
-      A[3] = mfma_i8(av[3], bv, A[3]);
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        v4i_t* A = acc[i][j];
+        A[0] = mfma_i8(av[i][0], bv, A[0]);
+        A[1] = mfma_i8(av[i][1], bv, A[1]);
+        A[2] = mfma_i8(av[i][2], bv, A[2]);
+        A[3] = mfma_i8(av[i][3], bv, A[3]);
+      }
       bv = *reinterpret_cast<const uint4*>(cp + BN * KS);
-      A[1] = mfma_i8(av[0], bv, A[1]);
-      A[2] = mfma_i8(av[1], bv, A[2]);
-      A[3] = mfma_i8(av[2], bv, A[3]);
-      A[4] = mfma_i8(av[3], bv, A[4]);
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        v4i_t* A = acc[i][j];
+        A[1] = mfma_i8(av[i][0], bv, A[1]);
+        A[2] = mfma_i8(av[i][1], bv, A[2]);
+        A[3] = mfma_i8(av[i][2], bv, A[3]);
+        A[4] = mfma_i8(av[i][3], bv, A[4]);
+      }
       bv = *reinterpret_cast<const uint4*>(cp + 2 * BN * KS);
-      A[2] = mfma_i8(av[0], bv, A[2]);
-      A[3] = mfma_i8(av[1], bv, A[3]);
-      A[4] = mfma_i8(av[2], bv, A[4]);
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        v4i_t* A = acc[i][j];
+        A[2] = mfma_i8(av[i][0], bv, A[2]);
+        A[3] = mfma_i8(av[i][1], bv, A[3]);
+        A[4] = mfma_i8(av[i][2], bv, A[4]);
+      }
       bv = *reinterpret_cast<const uint4*>(cp + 3 * BN * KS);
-      A[3] = mfma_i8(av[0], bv, A[3]);
-      A[4] = mfma_i8(av[1], bv, A[4]);
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        v4i_t* A = acc[i][j];
+        A[3] = mfma_i8(av[i][0], bv, A[3]);
+        A[4] = mfma_i8(av[i][1], bv, A[4]);
+      }
     }
     if (ks + 1 < nks) {
       NOPT_GLOAD_C((ks + 1) * KS);
@@ -205,73 +231,97 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #undef NOPT_GLOAD_A
 #undef NOPT_GLOAD_C
 #undef NOPT_LSTORE
-  if (xs.nseg == 0) {
+  // X values in FP64 (rows m0 + rowl(i, v), columns colb(j) of the group)
+  auto rowl = [&](int i, int v) { return wr * 16 * IT + i * 16 + 4 * lc + v; };
+  auto colb = [&](int j) { return wc * 16 * JT + j * 16 + lr; };
+  double xv[IT][JT][4];
+#pragma unroll
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int m = m0 + w * 16 + 4 * lc + v;
-      if (m < nbf) {
-        const int em = Be[Q * nbf + m];
-        double* xr = X + ((long)Q * nbf + m) * ldx;
+      const int m = m0 + rowl(i, v);
+      const int em = m < nbf ? Be[Q * nbf + m] : 0;
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-          const int col = cg * BN + j * 16 + lr;
-          const v4i_t* A = acc[j];
-          const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
-                            0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
-          xr[col] = ldexp(sv, em + Ce[col] - 12);
-        }
+      for (int j = 0; j < JT; ++j) {
+        const v4i_t* A = acc[i][j];
+        const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
+                          0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
+        xv[i][j][v] = m < nbf ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 12) : 0.0;
       }
     }
+  if (xs.nseg == 0) {
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + rowl(i, v);
+        if (m < nbf) {
+          double* xr = X + ((long)Q * nbf + m) * ldx + cg * BN;
+#pragma unroll
+          for (int j = 0; j < JT; ++j) xr[colb(j)] = xv[i][j][v];
+        }
+      }
     return;
   }
-  // fused digit split (one column group): X in FP64 registers, then per
-  // segment a row maximum over the segment's columns (16 lanes share a row),
-  // digits staged in LDS and written out as 16-byte chunks
-  double xv[NT][4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int m = m0 + w * 16 + 4 * lc + v;
-    const int em = m < nbf ? Be[Q * nbf + m] : 0;
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const v4i_t* A = acc[j];
-      const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
-                        0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
-      xv[j][v] = m < nbf ? ldexp(sv, em + Ce[j * 16 + lr] - 12) : 0.0;
-    }
-  }
+  // fused digit split (one column group): per segment a row maximum over the
+  // segment's columns (16 lanes of a wave, then the WC waves of a row via
+  // LDS), digits staged in LDS and written out as 16-byte chunks
   int8_t* st = &sm[0][0];
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
     const int ldk = (int)xs.ldk[d];
     const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
     for (int i = t; i < nch; i += 256) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
-    __syncthreads();
+    double rmx[IT][4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = w * 16 + 4 * lc + v;
-      const int m = m0 + row;
-      double mx = 0.0;
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
-      for (int j = 0; j < NT; ++j)
-        if (j * 16 >= o && j * 16 < o + pad) mx = fmax(mx, fabs(xv[j][v]));
+      for (int v = 0; v < 4; ++v) {
+        double mx = 0.0;
 #pragma unroll
-      for (int sh = 1; sh < 16; sh <<= 1) mx = fmax(mx, __shfl_xor(mx, sh, 64));
-      const int e = row_exponent(mx);
-      if (lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
+        for (int j = 0; j < JT; ++j) {
+          const int c0 = wc * 16 * JT + j * 16;
+          if (c0 >= o && c0 < o + pad) mx = fmax(mx, fabs(xv[i][j][v]));
+        }
 #pragma unroll
-      for (int j = 0; j < NT; ++j)
-        if (j * 16 >= o && j * 16 < o + pad) {
-          double r = ldexp(xv[j][v], 6 - e);
-          const int cidx = j * 16 + lr - o;
+        for (int sh = 1; sh < 16; sh <<= 1) mx = fmax(mx, __shfl_xor(mx, sh, 64));
+        if constexpr (WC == 1)
+          rmx[i][v] = mx;
+        else if (lr == 0)
+          rmax[wc][rowl(i, v)] = mx;
+      }
+    if constexpr (WC > 1) __syncthreads();
 #pragma unroll
-          for (int p = 0; p < 4; ++p) {
-            const double dg = rint(r);
-            st[(p * BMI + row) * ldk + cidx] = (int8_t)dg;
-            r = (r - dg) * 128.0;
+    for (int i = 0; i < IT; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = rowl(i, v);
+        const int m = m0 + row;
+        double mx;
+        if constexpr (WC == 1) {
+          mx = rmx[i][v];
+        } else {
+          mx = rmax[0][row];
+#pragma unroll
+          for (int q = 1; q < WC; ++q) mx = fmax(mx, rmax[q][row]);
+        }
+        const int e = row_exponent(mx);
+        if (wc == 0 && lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
+#pragma unroll
+        for (int j = 0; j < JT; ++j) {
+          const int c0 = wc * 16 * JT + j * 16;
+          if (c0 >= o && c0 < o + pad) {
+            double r = ldexp(xv[i][j][v], 6 - e);
+            const int cidx = c0 + lr - o;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+              const double dg = rint(r);
+              st[(p * BMI + row) * ldk + cidx] = (int8_t)dg;
+              r = (r - dg) * 128.0;
+            }
           }
         }
-    }
+      }
     __syncthreads();
     const int cpr = ldk / 16;  // chunks per row
     for (int i = t; i < nch; i += 256) {
@@ -374,16 +424,23 @@ __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
 // buffered in swizzled LDS with one barrier per slab; each thread prefetches
 // the row exponents of its own accumulator rows and columns.
 constexpr int KT8 = 64;
+//
+// Blocks are numbered so that each XCD (workgroups go to the 8 XCDs round
+// robin) owns a contiguous range of Q-splits with all their tiles: the tiles
+// of one split read the same X rows at about the same time, from that XCD's
+// L2. The grid is 1-D, ntiles * nsplit blocks, nsplit a multiple of 8.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_syrk_i8(
     const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
-    double* __restrict__ Kpart) {
+    int ntiles, int nsplit, double* __restrict__ Kpart) {
   constexpr int KS = 64, TB = 4 * KT8 * KS;  // bytes of one operand tile (4 planes)
   __shared__ __attribute__((aligned(16))) int8_t sm[2][2 * TB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
   const int lr = lane & 15, lc = lane >> 4;
+  const int per_xcd = ntiles * nsplit / 8;
+  const int gidx = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const int tile = gidx % ntiles, sp = gidx / ntiles, ns = nsplit;
   int bi, bj;
-  tri_tile(blockIdx.x, bi, bj);
-  const int sp = blockIdx.y, ns = gridDim.y;
+  tri_tile(tile, bi, bj);
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   const int nkq = (int)(ldk / KS);  // k-slabs per auxiliary function
   const int nsl = (q1 - q0) * nkq;
@@ -527,7 +584,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if (more) lstore(cur ^ 1);
     __syncthreads();
   }
-  double* Kt = Kpart + ((long)sp * gridDim.x + blockIdx.x) * KT8 * KT8;
+  double* Kt = Kpart + ((long)sp * ntiles + tile) * KT8 * KT8;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -560,9 +617,9 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
         xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] || !xs.xe[d])
       return hipErrorInvalidValue;
   switch (nt) {
-    case 2: k_half_i8<2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 4: k_half_i8<4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 6: k_half_i8<6><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 2: k_half_i8<2, 1><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 4: k_half_i8<4, 2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 6: k_half_i8<6, 1><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -582,9 +639,10 @@ hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, lo
   return hipGetLastError();
 }
 
-hipError_t launch_syrk_i8(dim3 grid, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
-                          int naux, double* Kpart) {
-  k_syrk_i8<<<grid, 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, Kpart);
+hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
+                          long ldk, int nbf, int naux, double* Kpart) {
+  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0) return hipErrorInvalidValue;
+  k_syrk_i8<<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
   return hipGetLastError();
 }
 

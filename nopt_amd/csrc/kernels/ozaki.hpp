@@ -33,8 +33,10 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
 // the five digit planes of B.
 hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                            int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride);
-// Exchange partials over 64x64 lower-triangle tiles (grid: tiles x splits).
+// Exchange partials over 64x64 lower-triangle tiles: Kpart[split][tile] for
+// ntiles tiles and nsplit Q-splits (a multiple of 8, one contiguous range of
+// splits per XCD).
 constexpr int kSyrkI8Tile = 64;
-hipError_t launch_syrk_i8(dim3 grid, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
-                          int naux, double* Kpart);
+hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
+                          long ldk, int nbf, int naux, double* Kpart);
 }  // namespace nopt
