@@ -72,8 +72,10 @@ KERNELS = {
     "half_transform": {"fp64": "k_half<double>", "fp32": "k_half<float>", "bf16x6": "k_half_bf16<3>",
                        "bf16x3": "k_half_bf16<2>", "int8": "k_half_i8"},
     "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
-              "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)", "int8": "k_gamma_i8"},
-    "coulomb_j": {p: "k_jpart<1>" for p in ("fp64", "fp32", "bf16x6", "bf16x3", "int8")},
+              "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)",
+              "int8": "k_gamma_slots<1, 2>"},
+    "coulomb_j": {"fp64": "k_jpart<1>", "fp32": "k_jpart<1>", "bf16x6": "k_jpart<1>", "bf16x3": "k_jpart<1>",
+                  "int8": "k_jpart_slots<1, 2>"},
     "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
                  "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8"},
 }
@@ -114,7 +116,7 @@ def roofline(phase, ms, nbf, naux, nocc, prec):
             return hbm(5.0 * naux * npair)
         return None  # partials ride on the half-transform
     if phase == "coulomb_j":
-        return hbm(8.0 * naux * npair)
+        return hbm((5.0 if prec == "int8" else 8.0) * naux * npair)
     return None
 
 

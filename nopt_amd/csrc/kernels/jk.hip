@@ -581,6 +581,11 @@ struct nopt_jk_ctx {
   void* Xe = nullptr;
   size_t Xe_bytes = 0;
   int ntiles8 = 0, nsk8 = 0;  // 64x64 exchange tiles of the integer path
+  // integer Coulomb pair: 8-column slots of the lower triangle, gamma
+  // partials per slot block, Q-splits of the gamma and J kernels
+  int* slots = nullptr;
+  int nslots = 0, nsg = 0, nsj8 = 0;
+  double* gslot = nullptr;
   // optional per-phase timing: events for up to kTimedBuilds builds
   bool timing = false;
   int timed = 0;
@@ -819,7 +824,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
   }
   // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
-  const int nsj = nj ? std::max(NSPLIT_J_MIN, (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj)) : 0;
+  const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(NSPLIT_J_MIN, (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj));
   if (nj) {
     size_t jb = c->Jpart_bytes;
     rc = ensure(&c->Jpart, &jb, sizeof(double) * nj * nsj * nn, "J partials");
@@ -844,8 +849,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (fused) {
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
     } else if (i8) {
-      CHK_(nopt::launch_gamma_i8(nj, naux, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, c->Dp, ldb, c->dstride,
-                                 c->gamma, naux));
+      CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux, c->slots,
+                                    c->nslots, c->Dp, ldb, c->dstride, c->gslot, c->gamma, naux));
     } else {
 #define NOPT_GF(ND)                                                                                         \
   case ND:                                                                                                  \
@@ -856,6 +861,9 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     CHK_(hipGetLastError());
     cmark(3);
+    if (i8)
+      return nopt::launch_jpart_slots(nj, nsj, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux, c->slots,
+                                      c->nslots, c->gamma, naux, Jpart, (long)nsj * nn);
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                     \
   case ND:                                                                                              \
@@ -1086,6 +1094,22 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   if (!rc) rc = dev_alloc((void**)&c->epart, sizeof(double) * 1024, "energy partials");
   if (!rc) rc = dev_alloc((void**)&c->dE, sizeof(double), "energy");
   if (!rc) rc = dev_alloc((void**)&c->info, sizeof(int), "info");
+  if (!rc) {
+    c->nslots = nopt::coulomb_slot_count((int)nbf);
+    std::vector<int> tab((size_t)c->nslots);
+    nopt::coulomb_slot_table((int)nbf, tab.data());
+    const int nsb = nopt::gamma_slot_blocks(c->nslots);
+    const int nbj = (c->nslots + 256 * nopt::kJSPL - 1) / (256 * nopt::kJSPL);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
+      ncu = 256;
+    c->nsg = (int)std::max(1L, std::min((long)naux, (4L * ncu + nsb - 1) / nsb));
+    c->nsj8 = (int)std::max(1L, std::min((long)naux, (4L * ncu + nbj - 1) / nbj));
+    rc = dev_alloc((void**)&c->slots, sizeof(int) * c->nslots, "Coulomb slot table");
+    if (!rc) rc = dev_alloc((void**)&c->gslot, sizeof(double) * nsb * kMaxBatch * naux, "gamma partials (slots)");
+    if (!rc && hipMemcpy(c->slots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess)
+      rc = set_error(NOPT_EHIP, "nopt_jk_create: copy of the slot table failed");
+  }
   if (rc) {
     nopt_jk_destroy(c);
     return rc;
@@ -1122,6 +1146,8 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->Ce);
   dev_free(c->Xd);
   dev_free(c->Xe);
+  dev_free(c->slots);
+  dev_free(c->gslot);
   for (int i = 0; i < kMaxBatch - 1; ++i) {
     dev_free(c->xD[i]);
     dev_free(c->xC[i]);

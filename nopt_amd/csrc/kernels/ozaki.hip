@@ -408,6 +408,147 @@ __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd,
     }
 }
 
+// ----------------------------------------------- Coulomb over triangle slots
+// The lower triangle of every B_Q is cut into 8-column slots (m, n0), n0 <= m,
+// listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the five
+// digit planes of a slot as 8-byte loads and rebuild the values exactly:
+//   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
+__device__ __forceinline__ void unpack8(const int8_t* b, long bplane, double (&v)[8]) {
+  int8_t dg[5][8];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
+    const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
+    v[j] = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
+  }
+}
+
+// gpart[sb][d][Q] = sum over the slots of slot block sb of B_Q,mn D_d,mn
+// (off-diagonal weight 2), Q in split blockIdx.y. Each wave takes one Q at a
+// time over all 64*SPL slots of the block (SPL per lane, the weighted density
+// values held in registers) and reduces across its lanes.
+template <int ND, int SPL>
+__global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bd, long bplane,
+                                                     const int* __restrict__ Be, long ld8, int nbf, int naux,
+                                                     const int* __restrict__ slots, int nslots,
+                                                     const double* __restrict__ Dp, long ldd, long dstride,
+                                                     double* __restrict__ gpart) {
+  const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int mrow[SPL], ncol[SPL];
+  double dv[ND][SPL][8];
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) {
+    const int sl = sb * 64 * SPL + k * 64 + lane;
+    const int code = sl < nslots ? slots[sl] : -1;
+    mrow[k] = code >= 0 ? code >> 16 : -1;
+    ncol[k] = code >= 0 ? code & 0xffff : 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int n = ncol[k] + j, m = mrow[k];
+        dv[d][k][j] = (m >= 0 && n <= m) ? (n < m ? 2.0 : 1.0) * Dp[d * dstride + (long)m * ldd + n] : 0.0;
+      }
+  }
+  for (int q = q0 + w; q < q1; q += 4) {
+    double acc[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) acc[d] = 0.0;
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      if (mrow[k] < 0) continue;
+      const long row = (long)q * nbf + mrow[k];
+      double bv[8];
+      unpack8(Bd + row * ld8 + ncol[k], bplane, bv);
+      const int sc = Be[row] - 6;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        double sacc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sacc += bv[j] * dv[d][k][j];
+        acc[d] += ldexp(sacc, sc);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      const double tot = warp_sum(acc[d]);
+      if (lane == 0) gpart[((long)sb * ND + d) * naux + q] = tot;
+    }
+  }
+}
+
+// gamma[d][Q] = sum_sb gpart[sb][d][Q]
+__global__ void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, int naux, double* __restrict__ gamma,
+                            long gstride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd * naux) return;
+  const int d = i / naux, q = i % naux;
+  double s = 0.0;
+  for (int b = 0; b < nsb; ++b) s += gpart[((long)b * nd + d) * naux + q];
+  gamma[d * gstride + q] = s;
+}
+
+// Jpart[d][split][m][n] = sum_{Q in split} gamma_d[Q] B_Q,mn over the slots:
+// each thread owns SPL slots (8 columns each) and accumulates over the Q of
+// its split in registers; B is rebuilt from its digit planes.
+template <int ND, int SPL>
+__global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bd, long bplane,
+                                                     const int* __restrict__ Be, long ld8, int nbf, int naux,
+                                                     const int* __restrict__ slots, int nslots,
+                                                     const double* __restrict__ gamma, long gstride,
+                                                     double* __restrict__ Jpart, long jstride) {
+  const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
+  const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
+  int mrow[SPL], ncol[SPL];
+  double acc[ND][SPL][8];
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) {
+    const int sl = sb * 256 * SPL + k * 256 + threadIdx.x;
+    const int code = sl < nslots ? slots[sl] : -1;
+    mrow[k] = code >= 0 ? code >> 16 : -1;
+    ncol[k] = code >= 0 ? code & 0xffff : 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[d][k][j] = 0.0;
+  }
+  for (int q = q0; q < q1; ++q) {
+    double g[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) g[d] = gamma[d * gstride + q];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      if (mrow[k] < 0) continue;
+      const long row = (long)q * nbf + mrow[k];
+      double bv[8];
+      unpack8(Bd + row * ld8 + ncol[k], bplane, bv);
+      const int sc = Be[row] - 6;
+#pragma unroll
+      for (int d = 0; d < ND; ++d) {
+        const double gs = ldexp(g[d], sc);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[d][k][j] += gs * bv[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) {
+    const int m = mrow[k];
+    if (m < 0) continue;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+      double* Js = Jpart + d * jstride + ((long)sp * nbf + m) * nbf;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (ncol[k] + j <= m) Js[ncol[k] + j] = acc[d][k][j];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ exchange
 __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
   bi = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -634,6 +775,50 @@ hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, lo
     case 2: k_gamma_i8<2, QB><<<nb, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
     case 3: k_gamma_i8<3, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
     case 4: k_gamma_i8<4, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int coulomb_slot_count(int nbf) {
+  long n = 0;
+  for (int m = 0; m < nbf; ++m) n += m / 8 + 1;
+  return (int)n;
+}
+
+void coulomb_slot_table(int nbf, int* out) {
+  long i = 0;
+  for (int m = 0; m < nbf; ++m)
+    for (int n0 = 0; n0 <= m; n0 += 8) out[i++] = (m << 16) | n0;
+}
+
+int gamma_slot_blocks(int nslots) { return (nslots + 64 * kGammaSPL - 1) / (64 * kGammaSPL); }
+
+hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
+                              long dstride, double* gpart, double* gamma, long gstride) {
+  const dim3 grid(gamma_slot_blocks(nslots), nsplit);
+  switch (nd) {
+    case 1: k_gamma_slots<1, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 2: k_gamma_slots<2, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 3: k_gamma_slots<3, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 4: k_gamma_slots<4, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    default: return hipErrorInvalidValue;
+  }
+  const int n = nd * naux;
+  k_gamma_sum<<<(n + 255) / 256, 256, 0, s>>>(gpart, (int)grid.x, nd, naux, gamma, gstride);
+  return hipGetLastError();
+}
+
+hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* gamma,
+                              long gstride, double* Jpart, long jstride) {
+  const dim3 grid((nslots + 256 * kJSPL - 1) / (256 * kJSPL), nsplit);
+  switch (nd) {
+    case 1: k_jpart_slots<1, kJSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 2: k_jpart_slots<2, kJSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 3: k_jpart_slots<3, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 4: k_jpart_slots<4, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

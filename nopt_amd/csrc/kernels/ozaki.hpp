@@ -33,6 +33,22 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
 // the five digit planes of B.
 hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                            int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride);
+// Coulomb pair from the five digit planes over 8-column slots (m, n0) of the
+// lower triangle: slot table (host), gamma (partials per slot block, then
+// summed) and J partials per Q-split (Jpart[d*jstride + split*nbf*nbf + m*nbf + n],
+// n <= m).
+constexpr int kGammaSPL = 2;  // slots per lane of the gamma kernel
+constexpr int kJSPL = 2;      // slots per thread of the J kernel
+int coulomb_slot_count(int nbf);
+void coulomb_slot_table(int nbf, int* out);
+int gamma_slot_blocks(int nslots);
+hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
+                              long dstride, double* gpart, double* gamma, long gstride);
+hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* gamma,
+                              long gstride, double* Jpart, long jstride);
+
 // Exchange partials over 64x64 lower-triangle tiles: Kpart[split][tile] for
 // ntiles tiles and nsplit Q-splits (a multiple of 8, one contiguous range of
 // splits per XCD).
