@@ -171,9 +171,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int lr = lane & 15, lc = lane >> 4;
   for (int ks = 0; ks < nks; ++ks) {
     const int cur = ks & 1;
-    // the B slab (HBM) is fetched before the products, the small C slab
-    // (L2-resident) after them, which keeps its registers free meanwhile
-    if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
+    // the next slab is fetched into registers before the products
+    if (ks + 1 < nks) {
+      NOPT_GLOAD_A((ks + 1) * KS);
+      NOPT_GLOAD_C((ks + 1) * KS);
+    }
     const int8_t* As = sm[cur];
     const int8_t* Cs = sm[cur] + AB;
     uint4 av[IT][4];
@@ -222,10 +224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         A[4] = mfma_i8(av[i][1], bv, A[4]);
       }
     }
-    if (ks + 1 < nks) {
-      NOPT_GLOAD_C((ks + 1) * KS);
-      NOPT_LSTORE(cur ^ 1);
-    }
+    if (ks + 1 < nks) NOPT_LSTORE(cur ^ 1);
     __syncthreads();
   }
 #undef NOPT_GLOAD_A
@@ -290,7 +289,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         else if (lr == 0)
           rmax[wc][rowl(i, v)] = mx;
       }
-    if constexpr (WC > 1) __syncthreads();
+    // orders the zeroing of the staging tile (and the row maxima of the
+    // other waves) before the digit writes below
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < IT; ++i)
 #pragma unroll
