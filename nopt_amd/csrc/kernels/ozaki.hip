@@ -171,11 +171,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int lr = lane & 15, lc = lane >> 4;
   for (int ks = 0; ks < nks; ++ks) {
     const int cur = ks & 1;
-    // the next slab is fetched into registers before the products
-    if (ks + 1 < nks) {
-      NOPT_GLOAD_A((ks + 1) * KS);
-      NOPT_GLOAD_C((ks + 1) * KS);
-    }
+    // the B slab (HBM) is fetched before the products, the small C slab
+    // (L2-resident) after them (measured faster than fetching both first)
+    if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
     const int8_t* As = sm[cur];
     const int8_t* Cs = sm[cur] + AB;
     uint4 av[IT][4];
@@ -224,7 +222,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         A[4] = mfma_i8(av[i][1], bv, A[4]);
       }
     }
-    if (ks + 1 < nks) NOPT_LSTORE(cur ^ 1);
+    if (ks + 1 < nks) {
+      NOPT_GLOAD_C((ks + 1) * KS);
+      NOPT_LSTORE(cur ^ 1);
+    }
     __syncthreads();
   }
 #undef NOPT_GLOAD_A
@@ -414,14 +415,19 @@ __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd,
 // listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the five
 // digit planes of a slot as 8-byte loads and rebuild the values exactly:
 //   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
-__device__ __forceinline__ void unpack8(const int8_t* b, long bplane, double (&v)[8]) {
-  int8_t dg[5][8];
+struct Dig8 {
+  uint2 p[5];  // 8 digits of each of the five planes
+};
+__device__ __forceinline__ void load_dig8(const int8_t* b, long bplane, Dig8& d) {
 #pragma unroll
-  for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
+  for (int k = 0; k < 5; ++k) d.p[k] = *reinterpret_cast<const uint2*>(b + k * bplane);
+}
+__device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
+  const int8_t* dg = reinterpret_cast<const int8_t*>(d.p);
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
-    const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
+    const int hi = 128 * (int)dg[j] + (int)dg[8 + j];
+    const int lo = 16384 * (int)dg[16 + j] + 128 * (int)dg[24 + j] + (int)dg[32 + j];
     v[j] = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
   }
 }
@@ -429,7 +435,8 @@ __device__ __forceinline__ void unpack8(const int8_t* b, long bplane, double (&v
 // gpart[sb][d][Q] = sum over the slots of slot block sb of B_Q,mn D_d,mn
 // (off-diagonal weight 2), Q in split blockIdx.y. Each wave takes one Q at a
 // time over all 64*SPL slots of the block (SPL per lane, the weighted density
-// values held in registers) and reduces across its lanes.
+// values held in registers) and reduces across its lanes; the digits of its
+// next Q are loaded while the current one is reduced.
 template <int ND, int SPL>
 __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bd, long bplane,
                                                      const int* __restrict__ Be, long ld8, int nbf, int naux,
@@ -445,39 +452,54 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
   for (int k = 0; k < SPL; ++k) {
     const int sl = sb * 64 * SPL + k * 64 + lane;
     const int code = sl < nslots ? slots[sl] : -1;
-    mrow[k] = code >= 0 ? code >> 16 : -1;
+    mrow[k] = code >= 0 ? code >> 16 : 0;  // row 0 stands in for an empty slot (zero weights)
     ncol[k] = code >= 0 ? code & 0xffff : 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int n = ncol[k] + j, m = mrow[k];
-        dv[d][k][j] = (m >= 0 && n <= m) ? (n < m ? 2.0 : 1.0) * Dp[d * dstride + (long)m * ldd + n] : 0.0;
+        dv[d][k][j] = (code >= 0 && n <= m) ? (n < m ? 2.0 : 1.0) * Dp[d * dstride + (long)m * ldd + n] : 0.0;
       }
   }
-  for (int q = q0 + w; q < q1; q += 4) {
+  Dig8 cur[SPL], nxt[SPL];
+  int ecur[SPL], enxt[SPL];
+  auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      const long row = (long)q * nbf + mrow[k];
+      load_dig8(Bd + row * ld8 + ncol[k], bplane, dst[k]);
+      e[k] = Be[row];
+    }
+  };
+  int q = q0 + w;
+  if (q < q1) fetch(q, cur, ecur);
+  for (; q < q1; q += 4) {
+    if (q + 4 < q1) fetch(q + 4, nxt, enxt);
     double acc[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) acc[d] = 0.0;
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
-      if (mrow[k] < 0) continue;
-      const long row = (long)q * nbf + mrow[k];
       double bv[8];
-      unpack8(Bd + row * ld8 + ncol[k], bplane, bv);
-      const int sc = Be[row] - 6;
+      unpack8(cur[k], bv);
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
         double sacc = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) sacc += bv[j] * dv[d][k][j];
-        acc[d] += ldexp(sacc, sc);
+        acc[d] += ldexp(sacc, ecur[k] - 6);
       }
     }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       const double tot = warp_sum(acc[d]);
       if (lane == 0) gpart[((long)sb * ND + d) * naux + q] = tot;
+    }
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      cur[k] = nxt[k];
+      ecur[k] = enxt[k];
     }
   }
 }
@@ -495,7 +517,8 @@ __global__ void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, i
 
 // Jpart[d][split][m][n] = sum_{Q in split} gamma_d[Q] B_Q,mn over the slots:
 // each thread owns SPL slots (8 columns each) and accumulates over the Q of
-// its split in registers; B is rebuilt from its digit planes.
+// its split in registers, loading the digits of the next Q while the current
+// one is accumulated.
 template <int ND, int SPL>
 __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bd, long bplane,
                                                      const int* __restrict__ Be, long ld8, int nbf, int naux,
@@ -505,41 +528,57 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
   const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   int mrow[SPL], ncol[SPL];
+  bool live[SPL];
   double acc[ND][SPL][8];
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
     const int sl = sb * 256 * SPL + k * 256 + threadIdx.x;
     const int code = sl < nslots ? slots[sl] : -1;
-    mrow[k] = code >= 0 ? code >> 16 : -1;
-    ncol[k] = code >= 0 ? code & 0xffff : 0;
+    live[k] = code >= 0;
+    mrow[k] = live[k] ? code >> 16 : 0;  // row 0 stands in for an empty slot (result not written)
+    ncol[k] = live[k] ? code & 0xffff : 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc[d][k][j] = 0.0;
   }
+  Dig8 cur[SPL], nxt[SPL];
+  int ecur[SPL], enxt[SPL];
+  auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      const long row = (long)q * nbf + mrow[k];
+      load_dig8(Bd + row * ld8 + ncol[k], bplane, dst[k]);
+      e[k] = Be[row];
+    }
+  };
+  if (q0 < q1) fetch(q0, cur, ecur);
   for (int q = q0; q < q1; ++q) {
+    if (q + 1 < q1) fetch(q + 1, nxt, enxt);
     double g[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) g[d] = gamma[d * gstride + q];
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
-      if (mrow[k] < 0) continue;
-      const long row = (long)q * nbf + mrow[k];
       double bv[8];
-      unpack8(Bd + row * ld8 + ncol[k], bplane, bv);
-      const int sc = Be[row] - 6;
+      unpack8(cur[k], bv);
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
-        const double gs = ldexp(g[d], sc);
+        const double gs = ldexp(g[d], ecur[k] - 6);
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[d][k][j] += gs * bv[j];
       }
     }
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+      cur[k] = nxt[k];
+      ecur[k] = enxt[k];
+    }
   }
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
+    if (!live[k]) continue;
     const int m = mrow[k];
-    if (m < 0) continue;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
       double* Js = Jpart + d * jstride + ((long)sp * nbf + m) * nbf;
