@@ -265,13 +265,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
   // fused digit split (one column group): per segment a row maximum over the
   // segment's columns (16 lanes of a wave, then the WC waves of a row via
-  // LDS); the digits of 4 neighbouring columns are packed into one dword by
-  // lane shuffles and stored straight to global memory
+  // LDS), digits staged in LDS and written out as 16-byte chunks
+  int8_t* st = &sm[0][0];
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
     const int ldk = (int)xs.ldk[d];
-    int8_t* xd = xs.xd[d];
-    const long xpl = xs.xplane[d];
+    const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
+    for (int i = t; i < nch; i += 256) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
     double rmx[IT][4];
 #pragma unroll
     for (int i = 0; i < IT; ++i)
@@ -285,50 +285,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
 #pragma unroll
         for (int sh = 1; sh < 16; sh <<= 1) mx = fmax(mx, __shfl_xor(mx, sh, 64));
-        rmx[i][v] = mx;
-        if constexpr (WC > 1)
-          if (lr == 0) rmax[wc][rowl(i, v)] = mx;
+        if constexpr (WC == 1)
+          rmx[i][v] = mx;
+        else if (lr == 0)
+          rmax[wc][rowl(i, v)] = mx;
       }
-    if constexpr (WC > 1) __syncthreads();
+    // orders the zeroing of the staging tile (and the row maxima of the
+    // other waves) before the digit writes below
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int m = m0 + rowl(i, v);
-        double mx = rmx[i][v];
-        if constexpr (WC > 1) {
-          mx = rmax[0][rowl(i, v)];
+        const int row = rowl(i, v);
+        const int m = m0 + row;
+        double mx;
+        if constexpr (WC == 1) {
+          mx = rmx[i][v];
+        } else {
+          mx = rmax[0][row];
 #pragma unroll
-          for (int q = 1; q < WC; ++q) mx = fmax(mx, rmax[q][rowl(i, v)]);
+          for (int q = 1; q < WC; ++q) mx = fmax(mx, rmax[q][row]);
         }
         const int e = row_exponent(mx);
         if (wc == 0 && lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
-        int8_t* xrow = xd + (Q * nbf + m) * (long)ldk;
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
           const int c0 = wc * 16 * JT + j * 16;
           if (c0 >= o && c0 < o + pad) {
             double r = ldexp(xv[i][j][v], 6 - e);
+            const int cidx = c0 + lr - o;
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
               const double dg = rint(r);
+              st[(p * BMI + row) * ldk + cidx] = (int8_t)dg;
               r = (r - dg) * 128.0;
-              // bytes of columns 4a..4a+3 gathered into lane 4a of the 16
-              const unsigned b0 = (unsigned)(int)dg & 0xffu;
-              const unsigned b1 = b0 | ((unsigned)__shfl_down((int)b0, 1, 16) << 8);
-              const unsigned b2 = b1 | ((unsigned)__shfl_down((int)b1, 2, 16) << 16);
-              if ((lr & 3) == 0 && m < nbf)
-                *reinterpret_cast<unsigned*>(xrow + p * xpl + (c0 - o) + lr) = b2;
             }
           }
         }
-        // zero columns [pad, ldk) of the segment
-        if (wc == WC - 1 && m < nbf)
-          for (int c = pad + 4 * lr; c < ldk; c += 64)
-#pragma unroll
-            for (int p = 0; p < 4; ++p) *reinterpret_cast<unsigned*>(xrow + p * xpl + c) = 0u;
       }
-    if constexpr (WC > 1) __syncthreads();  // rmax is reused by the next segment
+    __syncthreads();
+    const int cpr = ldk / 16;  // chunks per row
+    for (int i = t; i < nch; i += 256) {
+      const int p = i / (BMI * cpr), rem = i % (BMI * cpr), row = rem / cpr, ch = rem % cpr;
+      const int m = m0 + row;
+      if (m < nbf)
+        *reinterpret_cast<uint4*>(xs.xd[d] + p * xs.xplane[d] + (Q * nbf + m) * (long)ldk + 16 * ch) =
+            reinterpret_cast<const uint4*>(st)[i];
+    }
+    __syncthreads();
   }
 }
 
