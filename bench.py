@@ -69,8 +69,8 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
 
 # kernels of each timed phase, per mode
 KERNELS = {
-    "half_transform": {"fp64": "k_half<double>", "fp32": "k_half<float>", "bf16x6": "k_half_bf16<3>",
-                       "bf16x3": "k_half_bf16<2>", "int8": "k_half_i8"},
+    "half_transform": {"fp64": "k_half<double, 6>", "fp32": "k_half<float, 6>", "bf16x6": "k_half_bf16<3, 6>",
+                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 1>"},
     "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
               "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)",
               "int8": "k_gamma_slots<1, 2>"},
@@ -149,6 +149,99 @@ def perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3):
     return out
 
 
+def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
+    """The CASSCF-style ragged batch: `count` water clusters (H2O)_n, n ~
+    U{1..nmax} (seed), cc-pVDZ/cc-pVDZ-JKFIT, each with an inactive density
+    (all but 5 doubly occupied core-guess orbitals) and a total density with
+    an active space of 10 orbitals and a random PSD 1-RDM of trace 10
+    (factor [sqrt(2) C_core; NO sqrt(n/2)], src/molecule.cpp:2959-2977). All
+    2*count builds are captured in one HIP graph; builds/s of the replayed
+    graph and of the same builds launched one by one."""
+    from nopt_amd import geometry as G
+    from nopt_amd.molecule import Molecule
+    from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
+    sizes = G.batch_sizes(count, nmax, seed=seed)
+    t0 = time.perf_counter()
+    systems = []
+    for k, n in enumerate(sizes):
+        sym, xyz = G.water_cluster(n, seed=seed + 1 + k)
+        m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit")
+        S, T, V = m.one_electron()
+        s_, U = np.linalg.eigh(S)
+        X = U @ np.diag(s_ ** -0.5) @ U.T
+        C = X @ np.linalg.eigh(X.T @ (T + V) @ X)[1]
+        ncore = m.nocc - 5
+        rng = np.random.default_rng(seed + 1 + k)
+        A = rng.standard_normal((10, 10))
+        gam = A @ A.T
+        gam *= 10.0 / np.trace(gam)
+        occ, W = np.linalg.eigh(gam)
+        occ = np.clip(occ, 0.0, None)
+        Cc = np.ascontiguousarray(C[:, :ncore].T)
+        NO = W.T @ C[:, ncore:ncore + 10].T
+        Ct = np.ascontiguousarray(np.vstack([np.sqrt(2.0) * Cc, NO * np.sqrt(0.5 * occ)[:, None]]))
+        Dc = Cc.T @ Cc
+        Dt = Ct.T @ Ct
+        ctx = JKContext(m.nbf, m.naux)
+        ctx.set_B(m.eri3_packed(), m.metric())
+        d = dict(m=m, ctx=ctx, ncore=ncore, nt=Ct.shape[0], host=(Dc, Cc, Dt, Ct))
+        d["dev"] = [DeviceArray.from_host(a) if a.size else DeviceArray((1,)) for a in (Dc, Cc, Dt, Ct)]
+        d["out"] = [DeviceArray((m.nbf, m.nbf)) for _ in range(4)]
+        systems.append(d)
+    setup = time.perf_counter() - t0
+    stream = systems[0]["ctx"].stream()
+
+    def enqueue(p):
+        for d in systems:
+            dDc, dCc, dDt, dCt = d["dev"]
+            Jc, Kc, Jt, Kt = d["out"]
+            d["ctx"].build_batch_device([(dDc.ptr, dCc.ptr, d["ncore"], Jc.ptr, Kc.ptr),
+                                         (dDt.ptr, dCt.ptr, d["nt"], Jt.ptr, Kt.ptr)], p, stream)
+
+    res = {}
+    for p in ("fp64", prec):
+        for _ in range(warmup):
+            enqueue(p)
+        synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            enqueue(p)
+        synchronize()
+        eager = (time.perf_counter() - t1) / steps
+        ref = [[o.to_host() for o in d["out"]] for d in systems]
+        with Graph.capture(stream) as cap:
+            enqueue(p)
+        g = cap.graph
+        g.launch(stream)
+        synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            g.launch(stream)
+        synchronize()
+        graph = (time.perf_counter() - t1) / steps
+        same = max(float(np.abs(o.to_host() - r).max()) for d, rr in zip(systems, ref) for o, r in zip(d["out"], rr))
+        g.close()
+        res[p] = dict(eager_ms=1e3 * eager, graph_ms=1e3 * graph, builds_per_s_graph=2 * count / graph,
+                      builds_per_s_eager=2 * count / eager, graph_vs_eager_max_abs_diff=same, outputs=ref)
+    # accuracy of the mixed mode against FP64 over the whole batch
+    dE = dK = 0.0
+    for d, r64, rp in zip(systems, res["fp64"]["outputs"], res[prec]["outputs"]):
+        Dc, Cc, Dt, Ct = d["host"]
+        for (D, J64, K64, J, K) in ((Dc, r64[0], r64[1], rp[0], rp[1]), (Dt, r64[2], r64[3], rp[2], rp[3])):
+            dE = max(dE, abs(energy_terms(D, J, K) - energy_terms(D, J64, K64)))
+            dK = max(dK, float(np.abs(K - K64).max()))
+    for p in res:
+        res[p].pop("outputs")
+    for d in systems:
+        for a in d["dev"] + d["out"]:
+            a.free()
+        d["ctx"].close()
+    return dict(clusters=count, waters=int(sum(sizes)), sizes=sizes, builds_per_graph=2 * count,
+                nbf_range=[min(d["m"].nbf for d in systems), max(d["m"].nbf for d in systems)],
+                setup_s=round(setup, 1), modes={p: {k: round(v, 4) for k, v in r.items()} for p, r in res.items()},
+                max_abs_dE_vs_fp64=dE, max_abs_dK_vs_fp64=dK, precision=prec)
+
+
 def cpu_baseline(sym, xyz_ang, threads, nbuilds=3, label="C20H42"):
     """NOPT's CPU path (oracle restatement) on a bounded sample: one warm-up
     and `nbuilds` timed builds of the full C20H42 Cartesian workload."""
@@ -195,6 +288,7 @@ def main():
     ap.add_argument("--modes", default="fp64,int8,fp32,bf16x6,bf16x3",
                     help="comma-separated modes to time (fp64 is always built as the reference)")
     ap.add_argument("--no-perturbed", action="store_true", help="skip the perturbed-density accuracy check")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged water-cluster batch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -278,6 +372,22 @@ def main():
     dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec)), key=lambda k: phases[k])
     roof = roofline(dom, phases[dom], nbf, naux, nocc, prec)
     kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p) for k in timed} for p in modes}
+    # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) for this workload
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_c20h42.json")
+    if args.alkane == 20 and os.path.exists(tfile):
+        meas = json.load(open(tfile))["kernels"]
+        for p in modes:
+            for r in kern[p].values():
+                if r and r["kernel"] in meas:
+                    r["traffic"] = meas[r["kernel"]]["traffic_bytes"]
+        if roof["kernel"] in meas:
+            roof["traffic"] = meas[roof["kernel"]]["traffic_bytes"]
+
+    ragged = None
+    if not args.no_ragged:
+        ragged = ragged_batch(prec, max(3, args.steps // 5), 1)
+        log(f"[bench] ragged batch ({ragged['clusters']} clusters, {ragged['builds_per_graph']} builds per graph): "
+            f"{json.dumps(ragged['modes'])}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -320,6 +430,7 @@ def main():
                                                                   for kk, vv in v.items()})
                                       for k, v in kern[p].items()}} for p in modes},
             "perturbed_densities": pert,
+            "ragged_batch": ragged,
             "E_rhf_fp64": res.E,
             "setup_s": {"host_integrals": round(t_ints, 1), "B_device": round(t_B, 2), "scf": round(t_scf, 1)},
         },
