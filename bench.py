@@ -367,6 +367,15 @@ def main():
         prec = args.prec
     rate, ms, phases = runs[prec]
     rate64 = runs["fp64"][0]
+    # host-buffer path (D, C in and J, K out over PCIe, synchronous): reported, never the value
+    Ch = np.ascontiguousarray(Co)
+    ctx.build(D, Ch, nocc, prec=prec)
+    t0 = time.perf_counter()
+    nh = max(5, args.steps // 5)
+    for _ in range(nh):
+        ctx.build(D, Ch, nocc, prec=prec)
+    host_rate = nh / (time.perf_counter() - t0)
+    log(f"[bench] host-buffer path ({prec}): {host_rate:.1f} builds/s")
 
     timed = ("half_transform", "gamma", "coulomb_j", "exchange")
     dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec)), key=lambda k: phases[k])
@@ -423,6 +432,7 @@ def main():
             "max_abs_dK": acc[prec]["max_abs_dK"],
             "fp64_builds_per_s": round(rate64, 2),
             "speedup_vs_fp64": round(rate / rate64, 3),
+            "host_path_builds_per_s": round(host_rate, 2),
             "modes": {p: {"builds_per_s": round(runs[p][0], 2), "dE_vs_fp64_Eh": acc[p]["dE"],
                           "max_abs_dK": acc[p]["max_abs_dK"],
                           "phase_ms": {k: round(v, 4) for k, v in runs[p][2].items()},
