@@ -116,6 +116,11 @@ int nopt_jk_fock_energy(nopt_jk_ctx* ctx, const double* H, const double* D, cons
  * NULL. */
 int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, long nocc,
                             int prec, double* E, double* G, double* F);
+/* The same on device buffers, enqueued on `stream`: E is one device double;
+ * G (nocc x nbf) and F (nbf x nbf) may be NULL. The context's J, K and D
+ * buffers are used as scratch. */
+int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const double* dC, long nocc,
+                                   int prec, double* dE, double* dG, double* dF, void* stream);
 
 /* Per-phase timing with HIP events on the build's stream (no counterpart in
  * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When

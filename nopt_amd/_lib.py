@@ -62,6 +62,8 @@ GPU_SYMBOLS = [
                                            ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     ("nopt_jk_energy_gradient", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_int,
                                                _c_double_p, _c_double_p, _c_double_p]),
+    ("nopt_jk_energy_gradient_device", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_int, _vp, _vp, _vp,
+                                                      _vp]),
     ("nopt_jk_set_timing", ctypes.c_int, [_vp, ctypes.c_int]),
     ("nopt_jk_read_timing", ctypes.c_int, [_vp, _c_double_p, _c_long_p]),
     ("nopt_jk_stream", _vp, [_vp]),

@@ -1484,6 +1484,27 @@ int nopt_jk_fock_energy(nopt_jk_ctx* c, const double* H, const double* D, const 
   return NOPT_OK;
 }
 
+namespace {
+// E, G = 4 C F and F on device buffers, enqueued on s (the context's D, J, K
+// buffers serve as scratch). dG and dF may be null.
+int enqueue_energy_gradient(nopt_jk_ctx* c, const double* dH, const double* dC, long nocc, int prec, double* dE,
+                            double* dG, double* dF, hipStream_t s) {
+  const long nbf = c->nbf, nn = nbf * nbf;
+  double* F = dF ? dF : c->dF;
+  k_density<<<(unsigned)((nn + 255) / 256), 256, 0, s>>>(dC, (int)nocc, (int)nbf, c->dD);
+  HIPCHK(hipGetLastError());
+  int rc = enqueue_build(c, c->dD, dC, nocc, c->dJ, c->dK, prec, s);
+  if (rc) return rc;
+  const int nblk = (int)std::min<long>(1024, (nn + 255) / 256);
+  k_fock_energy<<<nblk, 256, 0, s>>>(dH, c->dD, c->dJ, c->dK, 2.0, 1.0, nn, F, c->epart);
+  k_sum<<<1, 256, 0, s>>>(c->epart, nblk, dE);
+  HIPCHK(hipGetLastError());
+  // G (nocc x nbf) = 4 C F
+  if (dG) HIPCHK(nopt::gemm_f64(false, (int)nocc, (int)nbf, (int)nbf, 4.0, dC, nbf, F, nbf, 0.0, dG, nbf, s));
+  return NOPT_OK;
+}
+}  // namespace
+
 int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, long nocc, int prec,
                             double* E, double* G, double* F) {
   nopt::clear_error();
@@ -1495,30 +1516,29 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, lo
   rc = upload(c, &c->dC, &c->dC_cap, C, nocc * nbf);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(c->dH, H, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-  k_density<<<(unsigned)((nn + 255) / 256), 256, 0, c->stream>>>(c->dC, (int)nocc, (int)nbf, c->dD);
-  HIPCHK(hipGetLastError());
-  rc = enqueue_build(c, c->dD, c->dC, nocc, c->dJ, c->dK, prec, c->stream);
-  if (rc) return rc;
-  const int nblk = (int)std::min<long>(1024, (nn + 255) / 256);
-  k_fock_energy<<<nblk, 256, 0, c->stream>>>(c->dH, c->dD, c->dJ, c->dK, 2.0, 1.0, nn, c->dF, c->epart);
-  k_sum<<<1, 256, 0, c->stream>>>(c->epart, nblk, c->dE);
-  HIPCHK(hipGetLastError());
-  if (G) {
-    if (!c->dG) {
-      rc = dev_alloc((void**)&c->dG, sizeof(double) * nn, "gradient");
-      if (rc) return rc;
-    }
-    // G (nocc x nbf) = 4 C F
-    HIPCHK(nopt::gemm_f64(false, (int)nocc, (int)nbf, (int)nbf, 4.0, c->dC, nbf, c->dF, nbf, 0.0, c->dG, nbf,
-                          c->stream));
-    HIPCHK(hipMemcpyAsync(G, c->dG, sizeof(double) * nocc * nbf, hipMemcpyDeviceToHost, c->stream));
+  if (G && !c->dG) {
+    rc = dev_alloc((void**)&c->dG, sizeof(double) * nn, "gradient");
+    if (rc) return rc;
   }
+  rc = enqueue_energy_gradient(c, c->dH, c->dC, nocc, prec, c->dE, G ? c->dG : nullptr, nullptr, c->stream);
+  if (rc) return rc;
+  if (G) HIPCHK(hipMemcpyAsync(G, c->dG, sizeof(double) * nocc * nbf, hipMemcpyDeviceToHost, c->stream));
   if (F) HIPCHK(hipMemcpyAsync(F, c->dF, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
   double e = 0.0;
   HIPCHK(hipMemcpyAsync(&e, c->dE, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (E) *E = e;
   return NOPT_OK;
+}
+
+int nopt_jk_energy_gradient_device(nopt_jk_ctx* c, const double* dH, const double* dC, long nocc, int prec,
+                                   double* dE, double* dG, double* dF, void* stream) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!dH || !dC || !dE || nocc <= 0 || nocc > c->nbf)
+    return set_error(NOPT_EINVAL, "nopt_jk_energy_gradient_device: H, C, E and 0 < nocc <= nbf are required");
+  return enqueue_energy_gradient(c, dH, dC, nocc, prec, dE, dG, dF, stream ? (hipStream_t)stream : c->stream);
 }
 
 }  // extern "C"
