@@ -79,8 +79,9 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* ctx);
 int nopt_jk_build(nopt_jk_ctx* ctx, const double* D, const double* C, long nocc, double* J,
                   double* K, int prec);
 
-/* The same build on device buffers, enqueued on `stream` (a hipStream_t, NULL
- * for the default stream); returns after enqueueing. */
+/* The same build on device buffers, enqueued on `stream` (a hipStream_t; NULL
+ * is the default (null) stream, nopt_jk_stream() the context's own); returns
+ * after enqueueing. */
 int nopt_jk_build_device(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc,
                          double* dJ, double* dK, int prec, void* stream);
 
@@ -96,7 +97,8 @@ int nopt_jk_build_device(nopt_jk_ctx* ctx, const double* dD, const double* dC, l
 #define NOPT_MAX_BATCH 4
 int nopt_jk_build_batch(nopt_jk_ctx* ctx, int nd, const double* const* D, const double* const* C,
                         const long* nocc, double* const* J, double* const* K, int prec);
-/* The batched build on device buffers, enqueued on `stream`. */
+/* The batched build on device buffers, enqueued on `stream` (NULL: the
+ * default stream). */
 int nopt_jk_build_batch_device(nopt_jk_ctx* ctx, int nd, const double* const* dD,
                                const double* const* dC, const long* nocc, double* const* dJ,
                                double* const* dK, int prec, void* stream);
@@ -116,7 +118,8 @@ int nopt_jk_fock_energy(nopt_jk_ctx* ctx, const double* H, const double* D, cons
  * NULL. */
 int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, long nocc,
                             int prec, double* E, double* G, double* F);
-/* The same on device buffers, enqueued on `stream`: E is one device double;
+/* The same on device buffers, enqueued on `stream` (NULL: the default
+ * stream): E is one device double;
  * G (nocc x nbf) and F (nbf x nbf) may be NULL. The context's J, K and D
  * buffers are used as scratch. */
 int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const double* dC, long nocc,

@@ -1375,7 +1375,7 @@ int nopt_jk_build_device(nopt_jk_ctx* c, const double* dD, const double* dC, lon
   nopt::clear_error();
   int rc = check_ctx(c, true);
   if (rc) return rc;
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
   return enqueue_build(c, dD, dC, nocc, dJ, dK, prec, s);
 }
 
@@ -1390,7 +1390,7 @@ int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, 
   for (int d = 0; d < nd; ++d)
     it[d] = BuildItem{dD ? dD[d] : nullptr, dC ? dC[d] : nullptr, nocc[d], dJ ? dJ[d] : nullptr,
                       dK ? dK[d] : nullptr};
-  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
   return enqueue_batch(c, nd, it, prec, s);
 }
 
@@ -1538,7 +1538,7 @@ int nopt_jk_energy_gradient_device(nopt_jk_ctx* c, const double* dH, const doubl
   if (rc) return rc;
   if (!dH || !dC || !dE || nocc <= 0 || nocc > c->nbf)
     return set_error(NOPT_EINVAL, "nopt_jk_energy_gradient_device: H, C, E and 0 < nocc <= nbf are required");
-  return enqueue_energy_gradient(c, dH, dC, nocc, prec, dE, dG, dF, stream ? (hipStream_t)stream : c->stream);
+  return enqueue_energy_gradient(c, dH, dC, nocc, prec, dE, dG, dF, (hipStream_t)stream);
 }
 
 }  // extern "C"

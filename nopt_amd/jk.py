@@ -135,7 +135,8 @@ class JKContext:
         return outs
 
     def build_device(self, dD, dC, nocc, dJ, dK, prec="fp64", stream=0):
-        """Device-pointer build (integer addresses), enqueued on `stream`."""
+        """Device-pointer build (integer addresses), enqueued on `stream`
+        (0: the default stream; `self.stream()`: the context's own)."""
         p = L.PREC[prec] if isinstance(prec, str) else int(prec)
         L.check(L.lib().nopt_jk_build_device(self._h, dD, dC, int(nocc), dJ, dK, p, stream or None))
 
