@@ -85,42 +85,43 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 
 // X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=4} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
 // for k, l < 4 (13 digit pairs; the products of equal k+l share one exact
-// int32 accumulator), combined in FP64. Tile: 64 rows x 16*NT columns; the
-// four waves form a (4/WC) x WC grid, each with IT = 4/(4/WC) row tiles and
+// int32 accumulator), combined in FP64. Tile: 64 rows x 16*NT columns; the NW
+// waves form a (NW/WC) x WC grid, each with IT = 4/(NW/WC) row tiles and
 // JT = NT/WC column tiles of 16, so that a wave reads IT + JT operand tiles
 // from LDS per 13*IT*JT products. k-slabs of 64 bytes, LDS double buffered
 // (one barrier per slab) with the next B slab prefetched into registers while
-// the current one is multiplied.
-template <int NT, int WC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_half_i8(
+// the current one is multiplied. NW = 8 halves the accumulators per wave so
+// that twice as many waves hide the memory latency.
+template <int NT, int WC, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_half_i8(
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
-  constexpr int BMI = 64, BN = 16 * NT, KS = 64;
-  constexpr int WR = 4 / WC, IT = 4 / WR, JT = NT / WC;
-  static_assert(NT % WC == 0, "column tiles must split evenly over the waves");
+  constexpr int BMI = 64, BN = 16 * NT, KS = 64, NTH = 64 * NW;
+  constexpr int WR = NW / WC, IT = 4 / WR, JT = NT / WC;
+  static_assert(NT % WC == 0 && NW % WC == 0 && IT * WR == 4, "tile split");
   constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per buffer
+  constexpr int APT = 1024 / NTH;                     // 16-byte B chunks per thread (4 planes x 64 rows x 4)
   constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
-  constexpr int CPT = CCH / 256;                      // = NT
-  static_assert(CCH % 256 == 0, "C slab must split evenly over the block");
+  constexpr int CPT = CCH / NTH;
+  static_assert(CCH % NTH == 0 && 1024 % NTH == 0, "slabs must split evenly over the block");
   __shared__ __attribute__((aligned(16))) int8_t sm[2][AB + CB];
-  __shared__ double rmax[WC > 1 ? WC : 1][WC > 1 ? BMI : 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wr = w / WC, wc = w % WC;
   const int m0 = blockIdx.x * BMI;
   const long Q = blockIdx.y;
   const int cg = blockIdx.z;
-  // A loader: row t/4, chunk t%4 of each of the 4 planes
-  const int ar = t >> 2, ac = t & 3;
+  // B loader: chunk e = t + NTH u is chunk e%4 of row (e%256)/4 in plane e/256
+  const int ar = (t & 255) >> 2, ac = t & 3;
   const bool arow_ok = m0 + ar < nbf;
   const int8_t* Ap = Bd + (Q * nbf + m0 + ar) * ld8 + 16 * ac;
   const int8_t* Cbase = Cd + (long)cg * BN * ld8;
-  // C loader: chunk e = t + 256 u of the slab is chunk (e & 3) of column
+  // C loader: chunk e = t + NTH u of the slab is chunk (e & 3) of column
   // (e >> 2) % BN in plane e / (4 BN); global and LDS offsets per u
   int c_g[CPT], c_s[CPT];
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
-    const int e = t + 256 * u;
+    const int e = t + NTH * u;
     const int c = e & 3, col = (e >> 2) % BN, p = e / (4 * BN);
     c_g[u] = (int)(p * cplane + (long)col * ld8 + 16 * c);
     c_s[u] = (p * BN + col) * KS + 16 * swz(col, c);
@@ -128,12 +129,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // prefetch registers; the C chunks sit in two arrays of at most 16 dwords
   // indexed with compile-time constants, so that both stay in registers
   constexpr int CH = (CPT + 1) / 2;
-  uint4 a_reg[4], c_lo[CH], c_hi[CPT - CH > 0 ? CPT - CH : 1];
+  uint4 a_reg[APT], c_lo[CH], c_hi[CPT - CH > 0 ? CPT - CH : 1];
 
 #define NOPT_GLOAD_A(k0)                                                                                 \
   do {                                                                                                   \
-    _Pragma("unroll") for (int p = 0; p < 4; ++p) a_reg[p] =                                             \
-        arow_ok ? *reinterpret_cast<const uint4*>(Ap + p * bplane + (k0)) : uint4{0, 0, 0, 0};           \
+    _Pragma("unroll") for (int u = 0; u < APT; ++u) a_reg[u] =                                           \
+        arow_ok ? *reinterpret_cast<const uint4*>(Ap + ((t + NTH * u) >> 8) * bplane + (k0))             \
+                : uint4{0, 0, 0, 0};                                                                     \
   } while (0)
 #define NOPT_GLOAD_C(k0)                                                                                 \
   do {                                                                                                   \
@@ -147,8 +149,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #define NOPT_LSTORE(buf)                                                                                 \
   do {                                                                                                   \
     int8_t* As_ = sm[buf];                                                                               \
-    _Pragma("unroll") for (int p = 0; p < 4; ++p)                                                        \
-        *reinterpret_cast<uint4*>(As_ + (p * BMI + ar) * KS + 16 * swz(ar, ac)) = a_reg[p];              \
+    _Pragma("unroll") for (int u = 0; u < APT; ++u)                                                      \
+        *reinterpret_cast<uint4*>(As_ + (((t + NTH * u) >> 8) * BMI + ar) * KS + 16 * swz(ar, ac)) =     \
+            a_reg[u];                                                                                    \
     static_for<CPT>([&](auto U) {                                                                        \
       constexpr int u_ = decltype(U)::value;                                                             \
       uint4* d_ = reinterpret_cast<uint4*>(As_ + AB + c_s[u_]);                                          \
@@ -266,12 +269,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   // fused digit split (one column group): per segment a row maximum over the
   // segment's columns (16 lanes of a wave, then the WC waves of a row via
   // LDS), digits staged in LDS and written out as 16-byte chunks
+  // the staging tile (at most 4 x 64 x 128 bytes) overlays the operand
+  // buffers; with WC > 1 it fits the first and the row maxima of the WC waves
+  // sit in the second (no LDS beyond the operand buffers, so that two blocks
+  // share a CU)
+  static_assert(4 * BMI * 128 <= (WC > 1 ? 1 : 2) * (AB + CB) && WC * BMI * 8 <= AB + CB, "epilogue LDS");
   int8_t* st = &sm[0][0];
+  double(*rmax)[BMI] = reinterpret_cast<double(*)[BMI]>(&sm[1][0]);
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
     const int ldk = (int)xs.ldk[d];
     const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
-    for (int i = t; i < nch; i += 256) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
+    for (int i = t; i < nch; i += NTH) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
     double rmx[IT][4];
 #pragma unroll
     for (int i = 0; i < IT; ++i)
@@ -326,7 +335,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
     __syncthreads();
     const int cpr = ldk / 16;  // chunks per row
-    for (int i = t; i < nch; i += 256) {
+    for (int i = t; i < nch; i += NTH) {
       const int p = i / (BMI * cpr), rem = i % (BMI * cpr), row = rem / cpr, ch = rem % cpr;
       const int m = m0 + row;
       if (m < nbf)
@@ -798,9 +807,9 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
         xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] || !xs.xe[d])
       return hipErrorInvalidValue;
   switch (nt) {
-    case 2: k_half_i8<2, 1><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 4: k_half_i8<4, 2><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 6: k_half_i8<6, 1><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 2: k_half_i8<2, 1, 4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 4: k_half_i8<4, 2, 8><<<grid, 512, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+    case 6: k_half_i8<6, 2, 8><<<grid, 512, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
