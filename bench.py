@@ -70,7 +70,7 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
 # kernels of each timed phase, per mode
 KERNELS = {
     "half_transform": {"fp64": "k_half<double, 6>", "fp32": "k_half<float, 6>", "bf16x6": "k_half_bf16<3, 6>",
-                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 1>"},
+                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 2, 8>"},
     "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
               "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)",
               "int8": "k_gamma_slots<1, 2>"},
@@ -242,6 +242,104 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
                 max_abs_dE_vs_fp64=dE, max_abs_dK_vs_fp64=dK, precision=prec)
 
 
+def large_system(n_water, seed, steps, threads):
+    """The large single system resident in HBM: a water cluster (H2O)_n
+    (seed, 1 g/cm^3, O-O >= 2.7 A) in cc-pVDZ / cc-pVDZ-JKFIT. B does not fit
+    twice in HBM (FP64 rows and digit planes), so each context keeps one form:
+    an FP64 context runs the reference RI-RHF to 1e-8 Eh from the core guess;
+    an integer context runs the same SCF with integer builds until the DIIS
+    error falls below 1e-5, then hands over to an FP64 context for the
+    end-game. Reports the energy difference of the two runs, the J/K/energy
+    differences of single builds on the converged density, and the builds/s
+    of both modes (device buffers, `steps` builds after one warm-up)."""
+    import contextlib
+    from nopt_amd import geometry as G
+    from nopt_amd.molecule import Molecule
+    from nopt_amd.jk import JKContext, DeviceArray, synchronize
+    from nopt_amd.scf import rhf
+    from nopt_amd import _lib as L
+    L.lib().nopt_set_host_threads(threads)
+    sym, xyz = G.water_cluster(n_water, seed)
+    mol = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit")
+    nbf, naux, nocc = mol.nbf, mol.naux, mol.nocc
+    log(f"[large] (H2O){n_water} seed {seed}: nbf {nbf} naux {naux} nocc {nocc}; host integrals ...")
+    t0 = time.perf_counter()
+    S, T, V = mol.one_electron()
+    H = T + V
+    metric = mol.metric()
+    eri3 = mol.eri3_packed()
+    t_ints = time.perf_counter() - t0
+    log(f"[large] host integrals {t_ints:.1f} s, packed (P|mn) {eri3.nbytes / 1e9:.1f} GB")
+    timings = {}
+
+    def context(keep):
+        t1 = time.perf_counter()
+        c = JKContext(nbf, naux, keep=keep)
+        c.set_B(eri3, metric)
+        dt = time.perf_counter() - t1
+        timings.setdefault(f"set_B_{keep}_s", []).append(round(dt, 1))
+        log(f"[large] {keep} context: B formed in {dt:.1f} s, {c.device_bytes() / 1e9:.1f} GB on the device")
+        return c
+
+    def rate(c, D, Co, prec):
+        dD, dC = DeviceArray.from_host(D), DeviceArray.from_host(Co)
+        dJ, dK = DeviceArray((nbf, nbf)), DeviceArray((nbf, nbf))
+        c.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
+        synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            c.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
+        synchronize()
+        dt = (time.perf_counter() - t1) / steps
+        out = (dJ.to_host(), dK.to_host())
+        for a in (dD, dC, dJ, dK):
+            a.free()
+        return 1.0 / dt, out
+
+    ctx = context("fp64")
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(sys.stderr):
+        ref = rhf(mol, ctx=ctx, H=H, S=S, prec="fp64", e_conv=1e-8, d_conv=1e-6, verbose=True)
+    t_ref = time.perf_counter() - t0
+    log(f"[large] FP64 RI-RHF E = {ref.E:.10f} in {ref.iters + 1} iterations, {t_ref:.1f} s")
+    Co = np.ascontiguousarray(ref.C[:, :nocc].T)
+    D = Co.T @ Co
+    rate64, (J64, K64) = rate(ctx, D, Co, "fp64")
+    log(f"[large] fp64: {rate64:.3f} builds/s")
+    ctx.close()
+    ctx = context("int8")
+    rate8, (J8, K8) = rate(ctx, D, Co, "int8")
+    dE_build = energy_terms(D, J8, K8) - energy_terms(D, J64, K64)
+    log(f"[large] int8: {rate8:.3f} builds/s; on the converged density dE {dE_build:.3e} "
+        f"max|dJ| {np.abs(J8 - J64).max():.3e} max|dK| {np.abs(K8 - K64).max():.3e}")
+
+    def to_fp64(c):
+        c.close()
+        return context("fp64")
+
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(sys.stderr):
+        mix = rhf(mol, ctx=ctx, H=H, S=S, prec="int8", e_conv=1e-8, d_conv=1e-6, verbose=True, switch_ctx=to_fp64)
+    t_mix = time.perf_counter() - t0
+    log(f"[large] mixed RI-RHF E = {mix.E:.10f} ({mix.builds}), dE vs FP64 run {mix.E - ref.E:.3e}, {t_mix:.1f} s")
+    return {
+        "metric": f"RI-RHF (H2O){n_water} cc-pVDZ/cc-pVDZ-JKFIT, integer builds with FP64 end-game vs FP64",
+        "value": round(rate8, 4), "unit": "builds/s", "n_gpus": 1, "higher_is_better": True, "dtype": "int8",
+        "data": f"water cluster seed {seed}, random orientations, 1 g/cm^3, O-O >= 2.7 A",
+        "config": {"workload": f"(H2O){n_water}", "nbf": nbf, "naux": naux, "nocc": nocc,
+                   "fp64_builds_per_s": round(rate64, 4), "int8_vs_fp64_build_speedup": round(rate8 / rate64, 3),
+                   "E_fp64": ref.E, "E_mixed": mix.E, "dE_scf": mix.E - ref.E,
+                   "scf_fp64": {"iterations": ref.iters + 1, "builds": ref.builds, "seconds": round(t_ref, 1)},
+                   "scf_mixed": {"iterations": mix.iters + 1, "builds": mix.builds, "switch_iter": mix.switch_iter,
+                                 "seconds": round(t_mix, 1)},
+                   "build_on_converged_density": {"dE": dE_build, "max_abs_dJ": float(np.abs(J8 - J64).max()),
+                                                  "max_abs_dK": float(np.abs(K8 - K64).max())},
+                   "host_integrals_s": round(t_ints, 1), **timings,
+                   "B_bytes": {"fp64_rows": 8 * naux * nbf * (((nbf + 1 + 15) // 16) * 16),
+                               "digit_planes": 5 * naux * nbf * (((nbf + 63) // 64) * 64)}},
+    }
+
+
 def cpu_baseline(sym, xyz_ang, threads, nbuilds=3, label="C20H42"):
     """NOPT's CPU path (oracle restatement) on a bounded sample: one warm-up
     and `nbuilds` timed builds of the full C20H42 Cartesian workload."""
@@ -292,6 +390,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--large", type=int, default=0, metavar="NWATER",
+                    help="instead of the headline, run the large-system RI-RHF on (H2O)_NWATER (seed 1) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (aux-index sharded) design is "
@@ -307,6 +407,10 @@ def main():
             L.check(L.lib().nopt_dev_memcpy(b.ptr, a.ptr, a.nbytes, 2))
         synchronize()
         print(json.dumps({"copy_probe_bytes": a.nbytes, "copies": 5}))
+        return
+
+    if args.large:
+        print(json.dumps(large_system(args.large, 1, max(3, min(args.steps, 10)), args.cpu_threads)))
         return
 
     from nopt_amd import geometry as G

@@ -51,9 +51,22 @@ typedef struct nopt_jk_ctx nopt_jk_ctx;
 int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out);
 void nopt_jk_destroy(nopt_jk_ctx* ctx);
 
+/* Which forms of B a context keeps. The default keeps both: FP64 rows (read
+ * by the FP64, FP32 and bf16 modes) and five int8 digit planes (read by the
+ * integer mode). A large system keeps one of them: FP64 only (8 bytes per
+ * element, full rows) or digit planes only (5 bytes per element); a build in
+ * a mode whose form is not kept fails with NOPT_ESTATE. NOPT has a single
+ * FP64 form of RI_AA (src/RI.cpp:16-32); this choice has no counterpart. */
+#define NOPT_CTX_DEFAULT 0
+#define NOPT_CTX_FP64_ONLY 1
+#define NOPT_CTX_INT8_ONLY 2
+int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** out);
+
 /* Forms the fitted tensor B = L^-1 (P|mn) on the device in FP64, where
  * V = L L^T is the Cholesky factor of the Coulomb metric, and keeps it
- * resident. eri3 is packed [P][m(m+1)/2+n] (m >= n), metric is naux x naux.
+ * resident. eri3 is packed [P][m(m+1)/2+n] (m >= n), metric is naux x naux;
+ * eri3 stays in host memory and passes through the device in chunks of rows m
+ * of at most 4 GiB (one pass, or two for a digit-planes-only context).
  * NOPT instead keeps raw (mn|P) with V^-1/2 from an eigendecomposition and
  * applies it on every build (src/RI.cpp:57-59, 152-156, src/matr.cpp:372-401);
  * any W with W^T W = V^-1 gives the same J and K. Fails with NOPT_ECHOL when

@@ -44,6 +44,8 @@ HOST_SYMBOLS = [
 ]
 GPU_SYMBOLS = [
     ("nopt_jk_create", ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.POINTER(_vp)]),
+    ("nopt_jk_create_ex", ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                         ctypes.POINTER(_vp)]),
     ("nopt_jk_destroy", None, [_vp]),
     ("nopt_jk_set_B", ctypes.c_int, [_vp, _c_double_p, _c_double_p]),
     ("nopt_jk_set_B_fitted", ctypes.c_int, [_vp, _c_double_p]),

@@ -45,23 +45,6 @@ constexpr int NSPLIT_J_MIN = 16; // minimum Q-split of the Coulomb pass
 constexpr int JCHUNK_MAX = 4096; // gamma values cached in LDS per split
 
 // ------------------------------------------------------------------ copies
-__global__ void k_unpack_B(const double* __restrict__ Bp, long npair, int nbf, long nrow_total,
-                           double* __restrict__ B64, long ld64) {
-  // one thread per (Q, m, n) with n < ld64
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nrow_total * ld64) return;
-  const long row = e / ld64;  // Q*nbf + m
-  const int n = (int)(e % ld64);
-  const long Q = row / nbf;
-  const int m = (int)(row % nbf);
-  double v = 0.0;
-  if (n < nbf) {
-    const int a = m > n ? m : n, b = m > n ? n : m;
-    v = Bp[Q * npair + (long)a * (a + 1) / 2 + b];
-  }
-  B64[row * ld64 + n] = v;
-}
-
 __global__ void k_pack_B(const double* __restrict__ B64, long ld64, int nbf, long npair, long naux,
                          double* __restrict__ Bp) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -545,6 +528,8 @@ struct nopt_jk_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double* B64 = nullptr;
+  int flags = 0;        // NOPT_CTX_*: which forms of B the context keeps
+  bool b_ready = false; // B formed
   // per-build work buffers (grown on demand)
   long cap_ldx = 0;
   void* X = nullptr;  // naux*nbf*ldx elements of the largest type
@@ -698,33 +683,75 @@ hipError_t launch_half(int nt, dim3 grid, hipStream_t s, const double* B, long l
 
 int check_ctx(nopt_jk_ctx* c, bool need_B) {
   if (!c) return set_error(NOPT_EINVAL, "null context");
-  if (need_B && !c->B64) return set_error(NOPT_ESTATE, "B tensor not set (call nopt_jk_set_B first)");
+  if (need_B && !c->b_ready) return set_error(NOPT_ESTATE, "B tensor not set (call nopt_jk_set_B first)");
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("hipSetDevice: ") + hip_msg(e));
   return NOPT_OK;
 }
 
-// Unpacks a packed fitted B on the device into B64.
-int finish_B(nopt_jk_ctx* c, const double* dBp) {
-  if (!c->B64) {
-    int rc = dev_alloc((void**)&c->B64, sizeof(double) * c->naux * c->nbf * c->ld64, "B (FP64)");
-    if (rc) return rc;
-  }
-  const long rows = c->naux * c->nbf;
-  const long total = rows * c->ld64;
-  k_unpack_B<<<(unsigned)((total + 255) / 256), 256, 0, c->stream>>>(dBp, c->npair, (int)c->nbf, rows,
-                                                                     c->B64, c->ld64);
-  HIPCHK(hipGetLastError());
-  // five int8 digit planes per row of B for the integer-split builds
-  if (!c->Bd) {
-    int rc = dev_alloc((void**)&c->Bd, (size_t)5 * rows * c->ld8, "B digit planes");
+// Forms B from packed lower-triangle rows in host memory, host[P][m(m+1)/2+n]:
+// raw (P|mn) solved against the Cholesky factor dL of the metric (B = L^-1
+// (P|mn)), or an already fitted tensor when dL is null. The rows m stream
+// through the device in chunks (whole rows, at most kChunkBytes), so the
+// packed tensor is never resident: each chunk is copied in, solved, and
+// written into the kept forms of B: FP64 rows, and the digit planes (split
+// from the FP64 rows, or, without them, from the chunks in a second pass once
+// every row maximum is known).
+constexpr size_t kChunkBytes = (size_t)4 << 30;
+
+int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
+  const bool keep64 = !(c->flags & NOPT_CTX_INT8_ONLY), keep8 = !(c->flags & NOPT_CTX_FP64_ONLY);
+  const long na = c->naux, nbf = c->nbf, rows = na * nbf;
+  const long cp = std::max(nbf, std::min(c->npair, (long)(kChunkBytes / (sizeof(double) * na))));
+  int rc = NOPT_OK;
+  c->b_ready = false;
+  if (keep64 && !c->B64) rc = dev_alloc((void**)&c->B64, sizeof(double) * rows * c->ld64, "B (FP64)");
+  if (!rc && keep8 && !c->Bd) {
+    rc = dev_alloc((void**)&c->Bd, (size_t)5 * rows * c->ld8, "B digit planes");
     if (!rc) rc = dev_alloc((void**)&c->Be, sizeof(int) * rows, "B row exponents");
-    if (rc) return rc;
   }
-  HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)c->nbf, rows, 5, c->Bd, c->ld8, rows, rows * c->ld8, c->Be,
-                             c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return NOPT_OK;
+  if (rc) return rc;
+  double* dA = nullptr;
+  unsigned long long* rmax = nullptr;
+  rc = dev_alloc((void**)&dA, sizeof(double) * na * cp, "chunk of the packed tensor");
+  if (!rc && keep8 && !keep64) rc = dev_alloc((void**)&rmax, sizeof(unsigned long long) * rows, "row maxima");
+  auto run = [&]() -> int {
+    hipStream_t s = c->stream;
+    if (keep64) HIPCHK(hipMemsetAsync(c->B64, 0, sizeof(double) * rows * c->ld64, s));
+    if (rmax) {
+      HIPCHK(hipMemsetAsync(c->Bd, 0, (size_t)5 * rows * c->ld8, s));
+      HIPCHK(hipMemsetAsync(rmax, 0, sizeof(unsigned long long) * rows, s));
+    }
+    // pass 1 forms the FP64 rows or the row maxima, pass 2 (digits without
+    // FP64 rows) the digit planes
+    for (int pass = 0; pass < (rmax ? 2 : 1); ++pass) {
+      for (int m0 = 0; m0 < (int)nbf;) {
+        int m1 = m0 + 1;
+        while (m1 < (int)nbf && (long)(m1 + 1) * (m1 + 2) / 2 - (long)m0 * (m0 + 1) / 2 <= cp) ++m1;
+        const long p0 = (long)m0 * (m0 + 1) / 2, w = (long)m1 * (m1 + 1) / 2 - p0;
+        HIPCHK(hipMemcpy2DAsync(dA, sizeof(double) * w, host + p0, sizeof(double) * c->npair, sizeof(double) * w,
+                                na, hipMemcpyHostToDevice, s));
+        if (dL) HIPCHK(nopt::trsm_lower_f64(dL, na, (int)na, dA, w, w, s));
+        if (keep64)
+          HIPCHK(nopt::unpack_packed_f64(dA, w, m0, m1, (int)nbf, na, c->B64, c->ld64, s));
+        else if (pass == 0)
+          HIPCHK(nopt::rowmax_packed(dA, w, m0, m1, (int)nbf, na, rmax, s));
+        else
+          HIPCHK(nopt::split_packed_i8(dA, w, m0, m1, (int)nbf, na, c->Be, 5, c->Bd, c->ld8, rows * c->ld8, s));
+        m0 = m1;
+      }
+      if (rmax && pass == 0) HIPCHK(nopt::rowmax_exponents(rmax, rows, c->Be, s));
+    }
+    if (keep64 && keep8)
+      HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)nbf, rows, 5, c->Bd, c->ld8, rows, rows * c->ld8, c->Be, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return NOPT_OK;
+  };
+  if (!rc) rc = run();
+  dev_free(dA);
+  dev_free(rmax);
+  if (!rc) c->b_ready = true;
+  return rc;
 }
 
 // One density of a batched build (device pointers). D: nbf x nbf; C: nocc x
@@ -755,6 +782,10 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
   const bool f64 = prec == NOPT_PREC_FP64, i8 = prec == NOPT_PREC_INT8;
   const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
+  if (i8 && !c->Bd)
+    return set_error(NOPT_ESTATE, "the context keeps no digit planes of B (NOPT_CTX_FP64_ONLY): integer mode unavailable");
+  if (!i8 && !c->B64)
+    return set_error(NOPT_ESTATE, "the context keeps no FP64 B (NOPT_CTX_INT8_ONLY): only the integer mode is available");
   const long unit = bf ? 32 : 16;  // column padding of each orbital block
   long off[kMaxBatch + 1];
   int jl[kMaxBatch];
@@ -1021,9 +1052,15 @@ int upload(nopt_jk_ctx* c, double** dst, long* cap, const double* src, long coun
 extern "C" {
 
 int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
+  return nopt_jk_create_ex(device, nbf, naux, NOPT_CTX_DEFAULT, out);
+}
+
+int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** out) {
   nopt::clear_error();
   if (!out) return set_error(NOPT_EINVAL, "nopt_jk_create: null output pointer");
   *out = nullptr;
+  if (flags != NOPT_CTX_DEFAULT && flags != NOPT_CTX_FP64_ONLY && flags != NOPT_CTX_INT8_ONLY)
+    return set_error(NOPT_EINVAL, "nopt_jk_create_ex: flags must be one of NOPT_CTX_DEFAULT, _FP64_ONLY, _INT8_ONLY");
   if (nbf <= 0 || naux <= 0) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf and naux must be positive");
   if (nbf > 2046) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf above 2046 is not supported");
   if (naux > 65535) return set_error(NOPT_EINVAL, "nopt_jk_create: naux above 65535 is not supported");
@@ -1034,6 +1071,7 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
   auto* c = new (std::nothrow) nopt_jk_ctx;
   if (!c) return set_error(NOPT_ENOMEM, "nopt_jk_create: host allocation failed");
   c->device = device;
+  c->flags = flags;
   c->nbf = nbf;
   c->naux = naux;
   c->npair = nbf * (nbf + 1) / 2;
@@ -1295,30 +1333,21 @@ int nopt_jk_set_B(nopt_jk_ctx* c, const double* eri3, const double* metric) {
   if (rc) return rc;
   if (!eri3 || !metric) return set_error(NOPT_EINVAL, "nopt_jk_set_B: null input");
   const long na = c->naux;
-  double *dV = nullptr, *dA = nullptr;
+  double* dV = nullptr;
   rc = dev_alloc((void**)&dV, sizeof(double) * na * na, "metric");
-  if (!rc) rc = dev_alloc((void**)&dA, sizeof(double) * na * c->npair, "raw 3-centre integrals");
   if (!rc) {
     hipError_t e = hipMemcpyAsync(dV, metric, sizeof(double) * na * na, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dA, eri3, sizeof(double) * na * c->npair, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = nopt::cholesky_f64(dV, na, (int)na, c->info, c->stream);
     int info = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&info, c->info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
+    if (e != hipSuccess)
       rc = set_error(NOPT_EHIP, std::string("metric factorisation: ") + hip_msg(e));
-    } else if (info != 0) {
-      rc = set_error(NOPT_ECHOL, "Coulomb metric is not positive definite (pivot " +
-                                     std::to_string(info - 1) + ")");
-    } else {
-      e = nopt::trsm_lower_f64(dV, na, (int)na, dA, c->npair, c->npair, c->stream);
-      if (e != hipSuccess) rc = set_error(NOPT_EHIP, std::string("triangular solve: ") + hip_msg(e));
-    }
+    else if (info != 0)
+      rc = set_error(NOPT_ECHOL, "Coulomb metric is not positive definite (pivot " + std::to_string(info - 1) + ")");
   }
-  if (!rc) rc = finish_B(c, dA);
+  if (!rc) rc = form_B(c, eri3, dV);
   dev_free(dV);
-  dev_free(dA);
   return rc;
 }
 
@@ -1327,15 +1356,7 @@ int nopt_jk_set_B_fitted(nopt_jk_ctx* c, const double* Bp) {
   int rc = check_ctx(c, false);
   if (rc) return rc;
   if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_set_B_fitted: null input");
-  double* dA = nullptr;
-  rc = dev_alloc((void**)&dA, sizeof(double) * c->naux * c->npair, "packed B");
-  if (!rc) {
-    hipError_t e = hipMemcpyAsync(dA, Bp, sizeof(double) * c->naux * c->npair, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) rc = set_error(NOPT_EHIP, std::string("copy B: ") + hip_msg(e));
-  }
-  if (!rc) rc = finish_B(c, dA);
-  dev_free(dA);
-  return rc;
+  return form_B(c, Bp, nullptr);
 }
 
 int nopt_jk_get_B(nopt_jk_ctx* c, double* Bp) {
@@ -1343,6 +1364,7 @@ int nopt_jk_get_B(nopt_jk_ctx* c, double* Bp) {
   int rc = check_ctx(c, true);
   if (rc) return rc;
   if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_get_B: null output");
+  if (!c->B64) return set_error(NOPT_ESTATE, "nopt_jk_get_B: the context keeps no FP64 B (NOPT_CTX_INT8_ONLY)");
   double* dA = nullptr;
   const long total = c->naux * c->npair;
   rc = dev_alloc((void**)&dA, sizeof(double) * total, "packed B");

@@ -20,6 +20,7 @@
 // Class sums are combined in FP64 and scaled by the row exponents.
 #include "ozaki.hpp"
 
+#include <type_traits>
 #include <utility>
 
 namespace nopt {
@@ -74,6 +75,90 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
       r = (r - d) * 128.0;
     }
   }
+}
+
+// ------------------------------------------- rows of packed symmetric chunks
+// A chunk of a symmetric tensor in packed lower-triangle form,
+// A[P*cp + m(m+1)/2 + n - m0(m0+1)/2] for the rows m0 <= m < m1 (n <= m), holds
+// each element of the full rows (P, m) and (P, n). A block takes a 64 x 64
+// tile (rows mb.., columns nb..) of one P through LDS, so that the element
+// (m, n) and its mirror (n, m) are both written along contiguous columns.
+// Modes: 0 writes FP64 rows (out64, ld), 1 folds |x| into the row maxima
+// (bit patterns of non-negative doubles order like their values), 2 writes
+// ns int8 digits with the rows' exponents (the digits k_split_rows forms).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_packed_tiles(const double* __restrict__ A, long cp, int m0, int m1, int nbf,
+                                                      double* __restrict__ out64, long ld,
+                                                      unsigned long long* __restrict__ rowmax,
+                                                      const int* __restrict__ expo, int ns,
+                                                      int8_t* __restrict__ planes, long pstride) {
+  __shared__ double tile[64][65];
+  const int nb = blockIdx.x * 64, mb = m0 + blockIdx.y * 64;
+  if (nb > mb + 63) return;  // above the diagonal: nothing stored (uniform per block)
+  const long P = blockIdx.z;
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long p0 = (long)m0 * (m0 + 1) / 2;
+  const double* Ap = A + P * cp;
+  double v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int r = w + 4 * k, m = mb + r, n = nb + c;
+    v[k] = (m < m1 && n <= m) ? Ap[(long)m * (m + 1) / 2 - p0 + n] : 0.0;
+    tile[r][c] = v[k];
+  }
+  // the element itself: row m, column n
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int m = mb + w + 4 * k, n = nb + c;
+    const bool ok = m < m1 && n <= m;
+    const long row = P * nbf + m;
+    if constexpr (MODE == 0) {
+      if (ok) out64[row * ld + n] = v[k];
+    } else if constexpr (MODE == 1) {
+      double mx = fabs(v[k]);
+      for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+      if (c == 0 && m < m1) atomicMax(rowmax + row, (unsigned long long)__double_as_longlong(mx));
+    } else {
+      if (ok) {
+        double r = ldexp(v[k], 6 - expo[row]);
+        for (int d = 0; d < ns; ++d) {
+          const double dg = rint(r);
+          planes[d * pstride + row * ld + n] = (int8_t)dg;
+          r = (r - dg) * 128.0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // the mirror: row n = nb + rr, column m = mb + c (strictly below the diagonal)
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int rr = w + 4 * k, n = nb + rr, m = mb + c;
+    const bool ok = m < m1 && n < m;
+    const double x = ok ? tile[c][rr] : 0.0;
+    const long row = P * nbf + n;
+    if constexpr (MODE == 0) {
+      if (ok) out64[row * ld + m] = x;
+    } else if constexpr (MODE == 1) {
+      double mx = fabs(x);
+      for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+      if (c == 0 && n < nbf) atomicMax(rowmax + row, (unsigned long long)__double_as_longlong(mx));
+    } else {
+      if (ok) {
+        double r = ldexp(x, 6 - expo[row]);
+        for (int d = 0; d < ns; ++d) {
+          const double dg = rint(r);
+          planes[d * pstride + row * ld + m] = (int8_t)dg;
+          r = (r - dg) * 128.0;
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_rowmax_exponent(const unsigned long long* __restrict__ rowmax, long rows, int* __restrict__ expo) {
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) expo[r] = row_exponent(__longlong_as_double((long long)rowmax[r]));
 }
 
 // ------------------------------------------------------------ half-transform
@@ -747,7 +832,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
     }
     if (qend) {
-      // combine the class sums of this auxiliary function in FP64
+      // combine the class sums of this auxiliary function: with |digit| <= 64
+      // and ldk <= 1024, hi = 128 A0 + A1 and mid = 128 A2 + A3 are exact in
+      // int32 and 2^21 hi + 2^7 mid + A4 (= 2^28 x the digit sum) in FP64
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -755,9 +842,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
           v4i_t* A = acc[i][j];
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
-                              0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
-            k64[i][j][v] += ldexp(sv, ea[i][v] + eb[j] - 12);
+            const int hi = A[0][v] * 128 + A[1][v], mid = A[2][v] * 128 + A[3][v];
+            const double sv = fma((double)hi, 0x1p21, fma((double)mid, 128.0, (double)A[4][v]));
+            k64[i][j][v] += ldexp(sv, ea[i][v] + eb[j] - 40);
           }
 #pragma unroll
           for (int c = 0; c < 5; ++c) A[c] = v4i_t{0, 0, 0, 0};
@@ -794,6 +881,44 @@ hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, in
   if (rows_dst <= 0) return hipSuccess;
   if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo) return hipErrorInvalidValue;
   k_split_rows<<<(unsigned)rows_dst, 64, 0, s>>>(src, ld, cols, rows_src, ns, planes, ldo, pstride, expo);
+  return hipGetLastError();
+}
+
+namespace {
+bool packed_chunk_ok(long cp, int m0, int m1, int nbf, long naux) {
+  return m0 >= 0 && m1 <= nbf && m0 < m1 && naux > 0 && naux <= 65535 &&
+         (long)m1 * (m1 + 1) / 2 - (long)m0 * (m0 + 1) / 2 <= cp;
+}
+dim3 packed_grid(int m0, int m1, long naux) { return dim3((m1 + 63) / 64, (m1 - m0 + 63) / 64, (unsigned)naux); }
+}  // namespace
+
+hipError_t unpack_packed_f64(const double* A, long cp, int m0, int m1, int nbf, long naux, double* B, long ld,
+                             hipStream_t s) {
+  if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || ld < nbf) return hipErrorInvalidValue;
+  k_packed_tiles<0><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, B, ld, nullptr, nullptr, 0,
+                                                               nullptr, 0);
+  return hipGetLastError();
+}
+
+hipError_t rowmax_packed(const double* A, long cp, int m0, int m1, int nbf, long naux, unsigned long long* rowmax,
+                         hipStream_t s) {
+  if (!packed_chunk_ok(cp, m0, m1, nbf, naux)) return hipErrorInvalidValue;
+  k_packed_tiles<1><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, nullptr, 0, rowmax, nullptr, 0,
+                                                               nullptr, 0);
+  return hipGetLastError();
+}
+
+hipError_t rowmax_exponents(const unsigned long long* rowmax, long rows, int* expo, hipStream_t s) {
+  if (rows > 0) k_rowmax_exponent<<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(rowmax, rows, expo);
+  return hipGetLastError();
+}
+
+hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* expo, int ns,
+                           int8_t* planes, long ldo, long pstride, hipStream_t s) {
+  if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || ldo < nbf || ns < 1 || ns > 8) return hipErrorInvalidValue;
+  if (pstride < naux * nbf * ldo) return hipErrorInvalidValue;
+  k_packed_tiles<2><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, nullptr, ldo, nullptr, expo, ns,
+                                                               planes, pstride);
   return hipGetLastError();
 }
 
@@ -875,7 +1000,7 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
 
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart) {
-  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0) return hipErrorInvalidValue;
+  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
   k_syrk_i8<<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
   return hipGetLastError();
 }

@@ -8,6 +8,21 @@ namespace nopt {
 // columns zero) and one exponent per row.
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
                          long rows_dst, long pstride, int* expo, hipStream_t s);
+// Full rows from a chunk of a symmetric tensor kept as packed lower
+// triangles, A[P*cp + m(m+1)/2 + n - m0(m0+1)/2] for the rows m0 <= m < m1
+// (n <= m) of every P < naux: each element goes to both full rows it belongs
+// to, (P, m) and (P, n). unpack_packed_f64 writes FP64 rows B[(P*nbf + r)*ld + col];
+// rowmax_packed folds the chunk into the row maxima (bit patterns, zeroed by
+// the caller), rowmax_exponents turns them into exponents, and
+// split_packed_i8 writes the digit planes of split_rows_i8 with those
+// exponents. Padding columns are the caller's to clear.
+hipError_t unpack_packed_f64(const double* A, long cp, int m0, int m1, int nbf, long naux, double* B, long ld,
+                             hipStream_t s);
+hipError_t rowmax_packed(const double* A, long cp, int m0, int m1, int nbf, long naux, unsigned long long* rowmax,
+                         hipStream_t s);
+hipError_t rowmax_exponents(const unsigned long long* rowmax, long rows, int* expo, hipStream_t s);
+hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* expo, int ns,
+                           int8_t* planes, long ldo, long pstride, hipStream_t s);
 // Digit split of the half-transformed rows fused into the half-transform:
 // segment d covers the columns [off[d], off[d] + len[d]) of X (off a multiple
 // of 16); its rows (Q, m) get their own exponent xe[d][Q*nbf + m] and four

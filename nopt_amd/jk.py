@@ -24,13 +24,20 @@ import numpy as np
 from . import _lib as L
 
 
+KEEP = {"both": 0, "fp64": 1, "int8": 2}  # NOPT_CTX_DEFAULT, _FP64_ONLY, _INT8_ONLY
+
+
 class JKContext:
-    def __init__(self, nbf, naux, device=0):
+    """A GPU build context. `keep` selects the forms of B held in HBM: "both"
+    (every mode), "fp64" (FP64/FP32/bf16 modes) or "int8" (integer mode)."""
+
+    def __init__(self, nbf, naux, device=0, keep="both"):
         lib = L.lib()
         if not hasattr(lib, "nopt_jk_create"):
             raise ImportError("libnoptjk.so was built without the HIP kernels")
         h = ctypes.c_void_p()
-        L.check(lib.nopt_jk_create(int(device), int(nbf), int(naux), ctypes.byref(h)))
+        L.check(lib.nopt_jk_create_ex(int(device), int(nbf), int(naux), KEEP[keep], ctypes.byref(h)))
+        self.keep = keep
         self._h = h
         self.nbf, self.naux = int(nbf), int(naux)
         self.npair = self.nbf * (self.nbf + 1) // 2

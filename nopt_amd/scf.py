@@ -9,9 +9,10 @@ Hueckel or SAD guess; this driver uses a core-Hamiltonian guess and DIIS on
 FC S - S C F... (the commutator FDS - SDF), which converges to the same fixed
 point. Orthogonalisation (S^-1/2), diagonalisation and the energy stay FP64.
 
-Mixed precision as iterative refinement: builds use `prec` (FP32 or BF16x3
-exchange) until the DIIS error falls below `switch_err`, then FP64 for the
-remaining iterations, so the converged energy is that of the FP64 build.
+Mixed precision as iterative refinement: builds use `prec` (integer split,
+FP32 or bf16 exchange) until the DIIS error falls below `switch_err`, then
+FP64 for the remaining iterations, so the converged energy is that of the
+FP64 build.
 """
 import time
 
@@ -25,9 +26,12 @@ class SCFResult(dict):
 
 
 def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, max_it=100,
-        diis_size=8, verbose=False, H=None, S=None):
+        diis_size=8, verbose=False, H=None, S=None, switch_ctx=None):
     """Returns SCFResult(E, C (nbf x nbf, columns are MOs), eps, D, F, iters,
-    builds {prec: count}, history)."""
+    builds {prec: count}, history, switch_iter). switch_ctx, if given, is
+    called with the current context when the FP64 end-game starts and returns
+    the context for the remaining builds (a large system that holds only the
+    integer form of B in HBM hands over to an FP64 context there)."""
     own = ctx is None
     if own:
         ctx = JKContext(mol.nbf, mol.naux)
@@ -46,6 +50,7 @@ def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, 
         E_old, D_old = 0.0, None
         fs, es = [], []
         builds, hist = {}, []
+        switch_iter = None
         for it in range(max_it):
             Co = C[:, :nocc]
             D = Co @ Co.T
@@ -65,6 +70,9 @@ def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, 
             if cur != "fp64" and (emax < switch_err or done):
                 cur = "fp64"  # FP64 end-game: refine with exact exchange
                 fs, es = [], []
+                switch_iter = it
+                if switch_ctx is not None:
+                    ctx = switch_ctx(ctx)
             E_old, D_old = E, D
             fs.append(F)
             es.append(err)
@@ -84,7 +92,7 @@ def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, 
             eps, Cp = np.linalg.eigh(X.T @ Fd @ X)
             C = X @ Cp
         return SCFResult(E=E, C=C, eps=eps, D=D, F=F, H=H, S=S, iters=it, builds=builds, history=hist,
-                         converged=done)
+                         converged=done, switch_iter=switch_iter)
     finally:
         if own:
             ctx.close()
