@@ -566,9 +566,15 @@ struct nopt_jk_ctx {
   void* Xe = nullptr;
   size_t Xe_bytes = 0;
   int ntiles8 = 0, nsk8 = 0;  // 64x64 exchange tiles of the integer path
-  // integer Coulomb pair: 8-column slots of the lower triangle, gamma
-  // partials per slot block, Q-splits of the gamma and J kernels
-  int* slots = nullptr;
+  // integer Coulomb pair: the lower triangle of B as packed rows in
+  // 16-column segments (five digit planes, plane stride splane), the segment
+  // and slot tables, gamma partials per slot block, Q-splits of the gamma
+  // and J kernels
+  int8_t* Bs = nullptr;
+  long splane = 0;
+  int nseg = 0;
+  int* segs = nullptr;   // segment table (m << 16 | n0), used to pack Bs
+  int* slots = nullptr;  // 8-column slots (m << 16 | n0) walked by the kernels
   int nslots = 0, nsg = 0, nsj8 = 0;
   double* gslot = nullptr;
   // optional per-phase timing: events for up to kTimedBuilds builds
@@ -709,6 +715,8 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
   if (!rc && keep8 && !c->Bd) {
     rc = dev_alloc((void**)&c->Bd, (size_t)5 * rows * c->ld8, "B digit planes");
     if (!rc) rc = dev_alloc((void**)&c->Be, sizeof(int) * rows, "B row exponents");
+    c->splane = na * 16L * c->nseg;
+    if (!rc) rc = dev_alloc((void**)&c->Bs, (size_t)5 * c->splane, "packed digit rows of B");
   }
   if (rc) return rc;
   double* dA = nullptr;
@@ -744,6 +752,8 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
     }
     if (keep64 && keep8)
       HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)nbf, rows, 5, c->Bd, c->ld8, rows, rows * c->ld8, c->Be, s));
+    if (keep8)
+      HIPCHK(nopt::pack_segments(c->Bd, rows * c->ld8, c->ld8, (int)nbf, na, c->segs, c->nseg, c->Bs, c->splane, s));
     HIPCHK(hipStreamSynchronize(s));
     return NOPT_OK;
   };
@@ -880,7 +890,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (fused) {
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
     } else if (i8) {
-      CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux, c->slots,
+      CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
                                     c->nslots, c->Dp, ldb, c->dstride, c->gslot, c->gamma, naux));
     } else {
 #define NOPT_GF(ND)                                                                                         \
@@ -893,7 +903,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     CHK_(hipGetLastError());
     cmark(3);
     if (i8)
-      return nopt::launch_jpart_slots(nj, nsj, sj, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux, c->slots,
+      return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
                                       c->nslots, c->gamma, naux, Jpart, (long)nsj * nn);
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                     \
@@ -906,8 +916,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   };
   auto coulomb_out = [&]() -> hipError_t {
     for (int k = 0; k < nj; ++k)
-      k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, sj>>>(Jpart + k * nsj * nn, nsj, nullptr, 0, 1, KT, nbf,
-                                                               it[jl[k]].J, nullptr);
+      k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, sj>>>(Jpart + (long)k * nsj * nn, nsj, nullptr, 0, 1, KT,
+                                                               nbf, it[jl[k]].J, nullptr);
     return hipGetLastError();
   };
   // 0: no J; 1: forked after staging; 2: in line (after the half-transform)
@@ -1133,6 +1143,9 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   if (!rc) rc = dev_alloc((void**)&c->dE, sizeof(double), "energy");
   if (!rc) rc = dev_alloc((void**)&c->info, sizeof(int), "info");
   if (!rc) {
+    c->nseg = nopt::coulomb_seg_count((int)nbf);
+    std::vector<int> stab((size_t)c->nseg);
+    nopt::coulomb_seg_table((int)nbf, stab.data());
     c->nslots = nopt::coulomb_slot_count((int)nbf);
     std::vector<int> tab((size_t)c->nslots);
     nopt::coulomb_slot_table((int)nbf, tab.data());
@@ -1144,9 +1157,11 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
     c->nsg = (int)std::max(1L, std::min((long)naux, (4L * ncu + nsb - 1) / nsb));
     c->nsj8 = (int)std::max(1L, std::min((long)naux, (4L * ncu + nbj - 1) / nbj));
     rc = dev_alloc((void**)&c->slots, sizeof(int) * c->nslots, "Coulomb slot table");
+    if (!rc) rc = dev_alloc((void**)&c->segs, sizeof(int) * c->nseg, "Coulomb segment table");
     if (!rc) rc = dev_alloc((void**)&c->gslot, sizeof(double) * nsb * kMaxBatch * naux, "gamma partials (slots)");
-    if (!rc && hipMemcpy(c->slots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess)
-      rc = set_error(NOPT_EHIP, "nopt_jk_create: copy of the slot table failed");
+    if (!rc && (hipMemcpy(c->slots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->segs, stab.data(), sizeof(int) * c->nseg, hipMemcpyHostToDevice) != hipSuccess))
+      rc = set_error(NOPT_EHIP, "nopt_jk_create: copy of the slot tables failed");
   }
   if (rc) {
     nopt_jk_destroy(c);
@@ -1185,7 +1200,9 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->Xd);
   dev_free(c->Xe);
   dev_free(c->slots);
+  dev_free(c->segs);
   dev_free(c->gslot);
+  dev_free(c->Bs);
   for (int i = 0; i < kMaxBatch - 1; ++i) {
     dev_free(c->xD[i]);
     dev_free(c->xC[i]);
@@ -1383,7 +1400,7 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   if (!c) return 0;
   long b = 0;
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
-  if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf;
+  if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf + 5L * c->splane;
   b += (long)(c->X_bytes + c->Cp_bytes + c->Cd_bytes + c->Ce_bytes + c->Xd_bytes + c->Xe_bytes + c->Jpart_bytes);
   b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux +
                          std::max((long)c->nsk * c->ntiles * KT * KT,

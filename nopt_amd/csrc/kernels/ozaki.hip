@@ -431,80 +431,43 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
   }
 }
 
-// ------------------------------------------------------------- Coulomb gamma
-// gamma[d][Q] = sum_mn B[Q][m][n] D_d[m][n] for ND densities (D_d at
-// Dp + d*dstride), over the lower triangle (off-diagonal weight 2) with B
-// rebuilt exactly from its five digit planes:
-//   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
-// One block per QB consecutive auxiliary functions (so that each density
-// chunk is read from cache once per QB functions); each wave walks row pairs
-// (p, nbf-1-p) in 8-column chunks.
-template <int ND, int QB>
-__global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd, long bplane,
-                                                  const int* __restrict__ Be, long ld8, int nbf, int naux,
-                                                  const double* __restrict__ Dp, long ldd, long dstride,
-                                                  double* __restrict__ gamma, long gstride) {
-  __shared__ double red[4];
-  const int qb0 = blockIdx.x * QB;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double acc[QB][ND];
-#pragma unroll
-  for (int q = 0; q < QB; ++q)
-#pragma unroll
-    for (int d = 0; d < ND; ++d) acc[q][d] = 0.0;
-  const int npairs = (nbf + 1) / 2;
-  for (int p = w; p < npairs; p += 4) {
-    const int m1 = p, m2 = nbf - 1 - p;
-    const int c1 = m1 / 8 + 1, c2 = (m2 != m1) ? m2 / 8 + 1 : 0;
-    for (int ch = lane; ch < c1 + c2; ch += 64) {
-      const int m = ch < c1 ? m1 : m2;
-      const int n0 = 8 * (ch < c1 ? ch : ch - c1);
-      double dv[ND][8];
-#pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        const double* dr = Dp + d * dstride + (long)m * ldd + n0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int n = n0 + j;
-          dv[d][j] = (n < m ? 2.0 : (n == m ? 1.0 : 0.0)) * dr[j];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < QB; ++q) {
-        if (qb0 + q >= naux) break;
-        const long row = (long)(qb0 + q) * nbf + m;
-        const int8_t* b = Bd + row * ld8 + n0;
-        int8_t dg[5][8];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) *reinterpret_cast<uint2*>(dg[k]) = *reinterpret_cast<const uint2*>(b + k * bplane);
-        double bv[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int hi = 128 * (int)dg[0][j] + (int)dg[1][j];
-          const int lo = 16384 * (int)dg[2][j] + 128 * (int)dg[3][j] + (int)dg[4][j];
-          bv[j] = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
-        }
-        const int sc = Be[row] - 6;
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-          double sacc = 0.0;
-#pragma unroll
-          for (int j = 0; j < 8; ++j) sacc += bv[j] * dv[d][j];
-          acc[q][d] += ldexp(sacc, sc);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < QB; ++q)
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-      const double tot = block_sum256(acc[q][d], red);
-      if (threadIdx.x == 0 && qb0 + q < naux) gamma[d * gstride + qb0 + q] = tot;
-    }
+// ------------------------------------------------- Coulomb over packed rows
+// The Coulomb pair reads B's lower triangle as packed rows cut into segments
+// of 16 columns: segment s = (row m, columns n0 .. n0+15), n0 a multiple of
+// 16, with the digits of row m (exponent Be[Q*nbf + m]) and zeros beyond
+// column m. Row m holds m/16 + 1 segments, starting at segment
+// seg0(m) = sum_{r<m} (r/16 + 1), and the segments of all rows follow each
+// other, so one B_Q is a contiguous run of sq = 16*nseg bytes per plane:
+// Bs[k*splane + Q*sq + 16*s + j]. seg_code[s] = (m << 16) | n0.
+__host__ __device__ __forceinline__ long seg0(int m) {
+  const long a = m / 16, b = m % 16;
+  return m + 8 * a * (a - 1) + a * b;
 }
 
-// ----------------------------------------------- Coulomb over triangle slots
+// Bs from the full digit rows: segment s of B_Q copied out of row (Q, m),
+// bytes beyond column m cleared. Grid: segments / 256 x naux.
+__global__ __launch_bounds__(256) void k_pack_segments(const int8_t* __restrict__ Bd, long bplane, long ld8, int nbf,
+                                                       const int* __restrict__ seg_code, int nseg,
+                                                       int8_t* __restrict__ Bs, long splane) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nseg) return;
+  const long Q = blockIdx.y;
+  const int code = seg_code[s], m = code >> 16, n0 = code & 0xffff;
+  const int valid = m - n0 + 1;  // columns of the row in this segment (1 .. 16)
+  const int8_t* src = Bd + (Q * nbf + m) * ld8 + n0;
+  int8_t* dst = Bs + Q * 16L * nseg + 16L * s;
+  for (int k = 0; k < 5; ++k) {
+    uint4 v = *reinterpret_cast<const uint4*>(src + k * bplane);
+    if (valid < 16) {
+      int8_t* b = reinterpret_cast<int8_t*>(&v);
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j >= valid) b[j] = 0;
+    }
+    *reinterpret_cast<uint4*>(dst + k * splane) = v;
+  }
+}
+
 // The lower triangle of every B_Q is cut into 8-column slots (m, n0), n0 <= m,
 // listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the five
 // digit planes of a slot as 8-byte loads and rebuild the values exactly:
@@ -512,9 +475,9 @@ __global__ __launch_bounds__(256) void k_gamma_i8(const int8_t* __restrict__ Bd,
 struct Dig8 {
   uint2 p[5];  // 8 digits of each of the five planes
 };
-__device__ __forceinline__ void load_dig8(const int8_t* b, long bplane, Dig8& d) {
+__device__ __forceinline__ void load_dig8(const int8_t* b, long splane, Dig8& d) {
 #pragma unroll
-  for (int k = 0; k < 5; ++k) d.p[k] = *reinterpret_cast<const uint2*>(b + k * bplane);
+  for (int k = 0; k < 5; ++k) d.p[k] = *reinterpret_cast<const uint2*>(b + k * splane);
 }
 __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
   const int8_t* dg = reinterpret_cast<const int8_t*>(d.p);
@@ -532,8 +495,8 @@ __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
 // values held in registers) and reduces across its lanes; the digits of its
 // next Q are loaded while the current one is reduced.
 template <int ND, int SPL>
-__global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bd, long bplane,
-                                                     const int* __restrict__ Be, long ld8, int nbf, int naux,
+__global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+                                                     const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
                                                      const double* __restrict__ Dp, long ldd, long dstride,
                                                      double* __restrict__ gpart) {
@@ -558,12 +521,14 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
   }
   Dig8 cur[SPL], nxt[SPL];
   int ecur[SPL], enxt[SPL];
+  int boff[SPL];  // byte offset of the slot in a packed B_Q
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
-      const long row = (long)q * nbf + mrow[k];
-      load_dig8(Bd + row * ld8 + ncol[k], bplane, dst[k]);
-      e[k] = Be[row];
+      load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+      e[k] = Be[(long)q * nbf + mrow[k]];
     }
   };
   int q = q0 + w;
@@ -614,8 +579,8 @@ __global__ void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, i
 // its split in registers, loading the digits of the next Q while the current
 // one is accumulated.
 template <int ND, int SPL>
-__global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bd, long bplane,
-                                                     const int* __restrict__ Be, long ld8, int nbf, int naux,
+__global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+                                                     const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
                                                      const double* __restrict__ gamma, long gstride,
                                                      double* __restrict__ Jpart, long jstride) {
@@ -638,12 +603,14 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
   }
   Dig8 cur[SPL], nxt[SPL];
   int ecur[SPL], enxt[SPL];
+  int boff[SPL];  // byte offset of the slot in a packed B_Q
+#pragma unroll
+  for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
 #pragma unroll
     for (int k = 0; k < SPL; ++k) {
-      const long row = (long)q * nbf + mrow[k];
-      load_dig8(Bd + row * ld8 + ncol[k], bplane, dst[k]);
-      e[k] = Be[row];
+      load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+      e[k] = Be[(long)q * nbf + mrow[k]];
     }
   };
   if (q0 < q1) fetch(q0, cur, ecur);
@@ -940,17 +907,19 @@ hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, lo
   return hipGetLastError();
 }
 
-hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                           int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride) {
-  constexpr int QB = 4;
-  const unsigned nb = (unsigned)((naux + QB - 1) / QB);
-  switch (nd) {
-    case 1: k_gamma_i8<1, QB><<<nb, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
-    case 2: k_gamma_i8<2, QB><<<nb, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
-    case 3: k_gamma_i8<3, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
-    case 4: k_gamma_i8<4, 2><<<(naux + 1) / 2, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Dp, ldd, dstride, gamma, gstride); break;
-    default: return hipErrorInvalidValue;
-  }
+int coulomb_seg_count(int nbf) { return (int)seg0(nbf); }
+
+void coulomb_seg_table(int nbf, int* out) {
+  long i = 0;
+  for (int m = 0; m < nbf; ++m)
+    for (int n0 = 0; n0 <= m; n0 += 16) out[i++] = (m << 16) | n0;
+}
+
+hipError_t pack_segments(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, const int* seg_code, int nseg,
+                         int8_t* Bs, long splane, hipStream_t s) {
+  if (naux <= 0 || naux > 65535 || splane < naux * 16L * nseg || ld8 % 16) return hipErrorInvalidValue;
+  k_pack_segments<<<dim3((nseg + 255) / 256, (unsigned)naux), 256, 0, s>>>(Bd, bplane, ld8, nbf, seg_code, nseg, Bs,
+                                                                           splane);
   return hipGetLastError();
 }
 
@@ -968,15 +937,15 @@ void coulomb_slot_table(int nbf, int* out) {
 
 int gamma_slot_blocks(int nslots) { return (nslots + 64 * kGammaSPL - 1) / (64 * kGammaSPL); }
 
-hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
+hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
+                              int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
                               long dstride, double* gpart, double* gamma, long gstride) {
   const dim3 grid(gamma_slot_blocks(nslots), nsplit);
   switch (nd) {
-    case 1: k_gamma_slots<1, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 2: k_gamma_slots<2, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 3: k_gamma_slots<3, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 4: k_gamma_slots<4, kGammaSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 1: k_gamma_slots<1, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 2: k_gamma_slots<2, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 3: k_gamma_slots<3, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 4: k_gamma_slots<4, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
     default: return hipErrorInvalidValue;
   }
   const int n = nd * naux;
@@ -984,15 +953,15 @@ hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
   return hipGetLastError();
 }
 
-hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* gamma,
+hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
+                              int nbf, int naux, const int* slots, int nslots, const double* gamma,
                               long gstride, double* Jpart, long jstride) {
   const dim3 grid((nslots + 256 * kJSPL - 1) / (256 * kJSPL), nsplit);
   switch (nd) {
-    case 1: k_jpart_slots<1, kJSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 2: k_jpart_slots<2, kJSPL><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 3: k_jpart_slots<3, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 4: k_jpart_slots<4, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 1: k_jpart_slots<1, kJSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 2: k_jpart_slots<2, kJSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 3: k_jpart_slots<3, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 4: k_jpart_slots<4, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

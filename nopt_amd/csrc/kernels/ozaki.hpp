@@ -44,24 +44,28 @@ struct XSplit {
 hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
                           int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx,
                           const XSplit& xs);
-// gamma[d][Q] = sum_mn B_Qmn D_d,mn (d < nd <= 4, D_d at Dp + d*dstride) from
-// the five digit planes of B.
-hipError_t launch_gamma_i8(int nd, int naux, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                           int nbf, const double* Dp, long ldd, long dstride, double* gamma, long gstride);
-// Coulomb pair from the five digit planes over 8-column slots (m, n0) of the
-// lower triangle: slot table (host), gamma (partials per slot block, then
-// summed) and J partials per Q-split (Jpart[d*jstride + split*nbf*nbf + m*nbf + n],
-// n <= m).
+// Coulomb pair over B's lower triangle in packed rows: row m of B_Q as
+// m/16 + 1 segments of 16 columns (zeros beyond column m), the rows one after
+// the other, five digit planes Bs[k*splane + Q*sq + 16*seg0(m) + n], sq =
+// 16*nseg (pack_segments copies them out of the full digit rows; seg_code[s]
+// = m << 16 | n0 for each segment). The kernels walk 8-column slots (m, n0)
+// of the triangle (slot table, m/8 + 1 slots per row): gamma (partials per
+// slot block, then summed) and J partials per Q-split
+// (Jpart[d*jstride + split*nbf*nbf + m*nbf + n], n <= m).
 constexpr int kGammaSPL = 2;  // slots per lane of the gamma kernel
 constexpr int kJSPL = 2;      // slots per thread of the J kernel
+int coulomb_seg_count(int nbf);
+void coulomb_seg_table(int nbf, int* out);
+hipError_t pack_segments(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, const int* seg_code, int nseg,
+                         int8_t* Bs, long splane, hipStream_t s);
 int coulomb_slot_count(int nbf);
 void coulomb_slot_table(int nbf, int* out);
 int gamma_slot_blocks(int nslots);
-hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
+hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
+                              int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
                               long dstride, double* gpart, double* gamma, long gstride);
-hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                              long ld8, int nbf, int naux, const int* slots, int nslots, const double* gamma,
+hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
+                              int nbf, int naux, const int* slots, int nslots, const double* gamma,
                               long gstride, double* Jpart, long jstride);
 
 // Exchange partials over 64x64 lower-triangle tiles: Kpart[split][tile] for
