@@ -331,10 +331,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
       const int em = m < nbf ? Be[Q * nbf + m] : 0;
 #pragma unroll
       for (int j = 0; j < JT; ++j) {
+        // 2^28 x the digit sum: hi = 128 A0 + A1 is exact in int32 (|A0| <= 4096 ld8,
+        // ld8 <= 2048) and every partial sum below 2^52 is exact in FP64
         const v4i_t* A = acc[i][j];
-        const double sv = (double)A[0][v] + 0x1p-7 * (double)A[1][v] + 0x1p-14 * (double)A[2][v] +
-                          0x1p-21 * (double)A[3][v] + 0x1p-28 * (double)A[4][v];
-        xv[i][j][v] = m < nbf ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 12) : 0.0;
+        const int hi = A[0][v] * 128 + A[1][v];
+        const double sv =
+            fma((double)hi, 0x1p21, fma((double)A[2][v], 0x1p14, fma((double)A[3][v], 128.0, (double)A[4][v])));
+        xv[i][j][v] = m < nbf ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
       }
     }
   if (xs.nseg == 0) {
@@ -351,40 +354,47 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
       }
     return;
   }
-  // fused digit split (one column group): per segment a row maximum over the
-  // segment's columns (16 lanes of a wave, then the WC waves of a row via
-  // LDS), digits staged in LDS and written out as 16-byte chunks
-  // the staging tile (at most 4 x 64 x 128 bytes) overlays the operand
-  // buffers; with WC > 1 it fits the first and the row maxima of the WC waves
-  // sit in the second (no LDS beyond the operand buffers, so that two blocks
-  // share a CU)
-  static_assert(4 * BMI * 128 <= (WC > 1 ? 1 : 2) * (AB + CB) && WC * BMI * 8 <= AB + CB, "epilogue LDS");
+  // fused digit split (one column group): per segment the row exponent (the
+  // largest frexp exponent over the segment's columns: 16 lanes of a wave,
+  // then the WC waves of a row via LDS), then each value as the integer
+  // N = rint(x 2^(27-e)), |N| <= 2^27, in balanced base-128 digits
+  // N = 2^21 d0 + 2^14 d1 + 2^7 d2 + d3 (|d| <= 64), staged in LDS and written
+  // out as 16-byte chunks.
+  // The staging tile (at most 4 x 64 x 128 bytes) overlays the operand
+  // buffers; with WC > 1 it fits the first and the row exponents of the WC
+  // waves sit in the second (no LDS beyond the operand buffers, so that two
+  // blocks share a CU)
+  static_assert(4 * BMI * 128 <= (WC > 1 ? 1 : 2) * (AB + CB) && WC * BMI * 4 <= AB + CB, "epilogue LDS");
+  constexpr int kNoExp = -4096;  // exponent of an all-zero row segment
   int8_t* st = &sm[0][0];
-  double(*rmax)[BMI] = reinterpret_cast<double(*)[BMI]>(&sm[1][0]);
+  int(*rexp)[BMI] = reinterpret_cast<int(*)[BMI]>(&sm[1][0]);
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
     const int ldk = (int)xs.ldk[d];
     const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
     for (int i = t; i < nch; i += NTH) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
-    double rmx[IT][4];
+    int rx[IT][4];
 #pragma unroll
     for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        double mx = 0.0;
+        int ex = kNoExp;
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
           const int c0 = wc * 16 * JT + j * 16;
-          if (c0 >= o && c0 < o + pad) mx = fmax(mx, fabs(xv[i][j][v]));
+          const double x = xv[i][j][v];
+          int k;
+          (void)frexp(x, &k);
+          if (c0 >= o && c0 < o + pad && x != 0.0) ex = max(ex, k);
         }
 #pragma unroll
-        for (int sh = 1; sh < 16; sh <<= 1) mx = fmax(mx, __shfl_xor(mx, sh, 64));
+        for (int sh = 1; sh < 16; sh <<= 1) ex = max(ex, __shfl_xor(ex, sh, 64));
         if constexpr (WC == 1)
-          rmx[i][v] = mx;
+          rx[i][v] = ex;
         else if (lr == 0)
-          rmax[wc][rowl(i, v)] = mx;
+          rexp[wc][rowl(i, v)] = ex;
       }
-    // orders the zeroing of the staging tile (and the row maxima of the
+    // orders the zeroing of the staging tile (and the row exponents of the
     // other waves) before the digit writes below
     __syncthreads();
 #pragma unroll
@@ -393,28 +403,32 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
       for (int v = 0; v < 4; ++v) {
         const int row = rowl(i, v);
         const int m = m0 + row;
-        double mx;
+        int e;
         if constexpr (WC == 1) {
-          mx = rmx[i][v];
+          e = rx[i][v];
         } else {
-          mx = rmax[0][row];
+          e = rexp[0][row];
 #pragma unroll
-          for (int q = 1; q < WC; ++q) mx = fmax(mx, rmax[q][row]);
+          for (int q = 1; q < WC; ++q) e = max(e, rexp[q][row]);
         }
-        const int e = row_exponent(mx);
+        if (e == kNoExp) e = 0;
         if (wc == 0 && lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
           const int c0 = wc * 16 * JT + j * 16;
           if (c0 >= o && c0 < o + pad) {
-            double r = ldexp(xv[i][j][v], 6 - e);
-            const int cidx = c0 + lr - o;
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-              const double dg = rint(r);
-              st[(p * BMI + row) * ldk + cidx] = (int8_t)dg;
-              r = (r - dg) * 128.0;
-            }
+            int n = (int)rint(ldexp(xv[i][j][v], 27 - e));
+            const int d3 = ((n + 64) & 127) - 64;
+            n = (n - d3) >> 7;
+            const int d2 = ((n + 64) & 127) - 64;
+            n = (n - d2) >> 7;
+            const int d1 = ((n + 64) & 127) - 64;
+            n = (n - d1) >> 7;
+            int8_t* sp = st + row * ldk + c0 + lr - o;
+            sp[0] = (int8_t)n;
+            sp[BMI * ldk] = (int8_t)d1;
+            sp[2 * BMI * ldk] = (int8_t)d2;
+            sp[3 * BMI * ldk] = (int8_t)d3;
           }
         }
       }
