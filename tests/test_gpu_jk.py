@@ -167,3 +167,33 @@ def test_nopt_ri_entry_points_match_oracle(c2h4_case):
         lib.regen_RI_AA_release()
     assert lib.DM_to_F_transform_RI(L.dptr(J), L.dptr(K), L.dptr(np.ascontiguousarray(D)), L.dptr(MO),
                                     om.nocc, n) < 0
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 2e-6), ("int8", 2e-6)])
+def test_nopt_orbital_file_through_the_gpu_build(prec, tol):
+    """NOPT's RHF orbitals of p-benzoquinone (its .orb file, GAMESS form,
+    Cartesian AOs): the GPU build at those orbitals gives the orbital energies
+    stored in the file, and the energy of the oracle's NOPT-path build."""
+    import oracle as O
+    from nopt_amd.orbfile import read_orb, gamess_to_libint
+    from systems import PBQ, PBQ_ORB
+    m = Molecule.from_angstrom(*PBQ, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
+    f = read_orb(PBQ_ORB)
+    C = gamess_to_libint(f.C, m.function_l())
+    S, T, V = m.one_electron()
+    H = T + V
+    Co = np.ascontiguousarray(C[:m.nocc])
+    D = Co.T @ Co
+    with JKContext(m.nbf, m.naux) as ctx:
+        ctx.set_B(m.eri3_packed(), m.metric())
+        J, K = ctx.build(D, Co, m.nocc, prec=prec)
+        E, _ = ctx.energy_gradient(H, Co, m.nocc, prec=prec)
+    F = H + 2 * J - K
+    eps = np.einsum("im,mn,in->i", C, F, C)
+    om = O.OracleMolecule.from_angstrom(PBQ[0], PBQ[1], os.path.join(BASIS, "cc-pvdz.gbs"),
+                                        os.path.join(BASIS, "cc-pvdz-jkfit.gbs"))
+    Jo, Ko = O.NoptJK(om.eri3_nopt(), O.s05(om.metric())[0], om.nocc)(D, Co, om.nocc)
+    E_o = float(np.sum(D * (H + 2 * Jo - Ko)) + np.sum(D * H) + om.e_nuc)
+    say(prec, "max|eps - eps_file|", np.abs(eps - f.eps).max(), "E", E + m.e_nuc, "oracle", E_o)
+    assert np.abs(eps - f.eps).max() < tol
+    assert abs(E + m.e_nuc - E_o) < (1e-9 if prec == "fp64" else 1e-6)
