@@ -156,7 +156,9 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
     an active space of 10 orbitals and a random PSD 1-RDM of trace 10
     (factor [sqrt(2) C_core; NO sqrt(n/2)], src/molecule.cpp:2959-2977). All
     2*count builds are captured in one HIP graph; builds/s of the replayed
-    graph and of the same builds launched one by one."""
+    graph and of the same builds launched one by one, with the split counts
+    of each system set by the autotuner (the replay with the counts picked at
+    context creation is reported beside it)."""
     from nopt_amd import geometry as G
     from nopt_amd.molecule import Molecule
     from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
@@ -198,6 +200,30 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
             d["ctx"].build_batch_device([(dDc.ptr, dCc.ptr, d["ncore"], Jc.ptr, Kc.ptr),
                                          (dDt.ptr, dCt.ptr, d["nt"], Jt.ptr, Kt.ptr)], p, stream)
 
+    def graph_ms(p):
+        enqueue(p)  # sizes every buffer before the capture
+        synchronize()
+        with Graph.capture(stream) as cap:
+            enqueue(p)
+        g = cap.graph
+        g.launch(stream)
+        synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            g.launch(stream)
+        synchronize()
+        g.close()
+        return 1e3 * (time.perf_counter() - t1) / steps
+
+    # the split counts of every system chosen by the autotuner (device builds
+    # of its total density), after a replay with the counts create picked
+    untuned = {p: graph_ms(p) for p in ("fp64", prec)}
+    t0 = time.perf_counter()
+    for d in systems:
+        dDc, dCc, dDt, dCt = d["dev"]
+        for p in ("fp64", prec):
+            d["ctx"].autotune(dDt.ptr, dCt.ptr, d["nt"], prec=p, reps=2)
+    tune_s = time.perf_counter() - t0
     res = {}
     for p in ("fp64", prec):
         for _ in range(warmup):
@@ -222,7 +248,8 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
         same = max(float(np.abs(o.to_host() - r).max()) for d, rr in zip(systems, ref) for o, r in zip(d["out"], rr))
         g.close()
         res[p] = dict(eager_ms=1e3 * eager, graph_ms=1e3 * graph, builds_per_s_graph=2 * count / graph,
-                      builds_per_s_eager=2 * count / eager, graph_vs_eager_max_abs_diff=same, outputs=ref)
+                      builds_per_s_eager=2 * count / eager, graph_vs_eager_max_abs_diff=same,
+                      graph_ms_default_splits=untuned[p], outputs=ref)
     # accuracy of the mixed mode against FP64 over the whole batch
     dE = dK = 0.0
     for d, r64, rp in zip(systems, res["fp64"]["outputs"], res[prec]["outputs"]):
@@ -236,7 +263,7 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
         for a in d["dev"] + d["out"]:
             a.free()
         d["ctx"].close()
-    return dict(clusters=count, waters=int(sum(sizes)), sizes=sizes, builds_per_graph=2 * count,
+    return dict(clusters=count, waters=int(sum(sizes)), sizes=sizes, builds_per_graph=2 * count, autotune_s=round(tune_s, 1),
                 nbf_range=[min(d["m"].nbf for d in systems), max(d["m"].nbf for d in systems)],
                 setup_s=round(setup, 1), modes={p: {k: round(v, 4) for k, v in r.items()} for p, r in res.items()},
                 max_abs_dE_vs_fp64=dE, max_abs_dK_vs_fp64=dK, precision=prec)

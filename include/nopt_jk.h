@@ -138,6 +138,22 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* ctx, const double* H, const double* C, 
 int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const double* dC, long nocc,
                                    int prec, double* dE, double* dG, double* dF, void* stream);
 
+/* Tile and split tuning. The build divides the auxiliary index into Q-splits
+ * (partial sums over Q ranges, added in FP64 at the end): params[5] =
+ * {FP64/FP32/bf16 exchange splits, FP64 J splits (0: the fewest the kernel's
+ * gamma cache allows, at least 16), integer exchange splits (a multiple of
+ * 8), integer gamma splits, integer J splits}. create picks them from the
+ * device's CU count; nopt_jk_autotune times `reps` device builds of (dD, dC)
+ * in mode `prec` on `stream` for each candidate of each split count in turn
+ * (coordinate descent), keeps the fastest and returns its mean milliseconds
+ * per build in *best_ms (may be NULL). Results differ from the untuned build
+ * only in the order of the FP64 partial sums. No counterpart in NOPT, whose
+ * DGEMMs split rows evenly over threads (include/blas_link.h:54-116). */
+int nopt_jk_get_tuning(const nopt_jk_ctx* ctx, int* params);
+int nopt_jk_set_tuning(nopt_jk_ctx* ctx, const int* params);
+int nopt_jk_autotune(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc, int prec, int reps,
+                     double* best_ms, void* stream);
+
 /* Per-phase timing with HIP events on the build's stream (no counterpart in
  * NOPT, whose RHF loop disables its step timers, src/SCF.cpp:124). When
  * enabled, each build (or batch) records events around its six phases:

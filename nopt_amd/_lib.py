@@ -46,6 +46,9 @@ GPU_SYMBOLS = [
     ("nopt_jk_create", ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.POINTER(_vp)]),
     ("nopt_jk_create_ex", ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long, ctypes.c_int,
                                          ctypes.POINTER(_vp)]),
+    ("nopt_jk_get_tuning", ctypes.c_int, [_vp, _c_int_p]),
+    ("nopt_jk_set_tuning", ctypes.c_int, [_vp, _c_int_p]),
+    ("nopt_jk_autotune", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, _c_double_p, _vp]),
     ("nopt_jk_destroy", None, [_vp]),
     ("nopt_jk_set_B", ctypes.c_int, [_vp, _c_double_p, _c_double_p]),
     ("nopt_jk_set_B_fitted", ctypes.c_int, [_vp, _c_double_p]),

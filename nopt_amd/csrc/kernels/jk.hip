@@ -538,6 +538,8 @@ struct nopt_jk_ctx {
   void* Jpart = nullptr;  // J partials: (densities x splits) x nbf x nbf
   size_t Jpart_bytes = 0;
   int nsk = 0;
+  int nsj = 0;  // Q-splits of the FP64 J partials (0: the smallest the LDS gamma cache allows, at least 16)
+  size_t Kpart_bytes = 0;
   double* Dp = nullptr;  // padded densities, kMaxBatch slots of dstride
   long dstride = 0;
   void* Cp = nullptr;    // padded orbital factor
@@ -865,7 +867,16 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
   }
   // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
-  const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(NSPLIT_J_MIN, (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj));
+  const int nsj_min = !nj ? 0 : (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj);
+  const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(nsj_min, c->nsj > 0 ? c->nsj : NSPLIT_J_MIN);
+  {  // K partials of the current split counts
+    size_t kb = c->Kpart_bytes;
+    const size_t need = sizeof(double) * (i8 ? (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
+                                              : (size_t)c->nsk * c->ntiles * KT * KT);
+    rc = ensure((void**)&c->Kpart, &kb, need, "K partials");
+    if (rc) return rc;
+    c->Kpart_bytes = kb;
+  }
   if (nj) {
     size_t jb = c->Jpart_bytes;
     rc = ensure(&c->Jpart, &jb, sizeof(double) * nj * nsj * nn, "J partials");
@@ -1127,11 +1138,11 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   const long nn = nbf * nbf;
   if (!rc) rc = dev_alloc((void**)&c->gpart, sizeof(double) * naux * c->mtiles64, "gamma partials");
   if (!rc) rc = dev_alloc((void**)&c->gamma, sizeof(double) * kMaxBatch * naux, "gamma");
-  if (!rc)
-    rc = dev_alloc((void**)&c->Kpart,
-                   sizeof(double) * std::max((long)c->nsk * c->ntiles * KT * KT,
-                                             (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile),
-                   "K partials");
+  if (!rc) {
+    c->Kpart_bytes = sizeof(double) * std::max((long)c->nsk * c->ntiles * KT * KT,
+                                               (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile);
+    rc = dev_alloc((void**)&c->Kpart, c->Kpart_bytes, "K partials");
+  }
   c->dstride = nbf * std::max(c->ld64, c->ld8);
   if (!rc) rc = dev_alloc((void**)&c->Dp, sizeof(double) * kMaxBatch * c->dstride, "padded densities");
   if (!rc) rc = dev_alloc((void**)&c->dJ, sizeof(double) * nn, "J");
@@ -1216,6 +1227,109 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+int nopt_jk_get_tuning(const nopt_jk_ctx* c, int* params) {
+  nopt::clear_error();
+  if (!c || !params) return set_error(NOPT_EINVAL, "nopt_jk_get_tuning: null argument");
+  params[0] = c->nsk;
+  params[1] = c->nsj;
+  params[2] = c->nsk8;
+  params[3] = c->nsg;
+  params[4] = c->nsj8;
+  return NOPT_OK;
+}
+
+int nopt_jk_set_tuning(nopt_jk_ctx* c, const int* params) {
+  nopt::clear_error();
+  int rc = check_ctx(c, false);
+  if (rc) return rc;
+  if (!params) return set_error(NOPT_EINVAL, "nopt_jk_set_tuning: null argument");
+  const int na = (int)c->naux;
+  if (params[0] < 1 || params[0] > na || params[1] < 0 || params[1] > na || params[2] < 8 || params[2] % 8 ||
+      params[2] > std::max(8, na) || params[3] < 1 || params[3] > na || params[4] < 1 || params[4] > na)
+    return set_error(NOPT_EINVAL, "nopt_jk_set_tuning: split counts out of range (see nopt_jk.h)");
+  c->nsk = params[0];
+  c->nsj = params[1];
+  c->nsk8 = params[2];
+  c->nsg = params[3];
+  c->nsj8 = params[4];
+  return NOPT_OK;
+}
+
+int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, int prec, int reps,
+                     double* best_ms, void* stream) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!dD || !dC || reps < 1) return set_error(NOPT_EINVAL, "nopt_jk_autotune: null input or reps < 1");
+  if (t_capturing) return set_error(NOPT_ESTATE, "nopt_jk_autotune: not during graph capture");
+  hipStream_t s = (hipStream_t)stream;
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  // mean time of `reps` builds of the current settings, after one warm-up
+  auto measure = [&](float* ms) -> int {
+    int r = enqueue_build(c, dD, dC, nocc, c->dJ, c->dK, prec, s);
+    if (r) return r;
+    HIPCHK(hipEventRecord(e0, s));
+    for (int i = 0; i < reps && !r; ++i) r = enqueue_build(c, dD, dC, nocc, c->dJ, c->dK, prec, s);
+    if (r) return r;
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventElapsedTime(ms, e0, e1));
+    *ms /= reps;
+    return NOPT_OK;
+  };
+  const int na = (int)c->naux;
+  const bool i8 = prec == NOPT_PREC_INT8;
+  // coordinate descent over the split counts the mode uses
+  int* knobs[3];
+  std::vector<int> cand[3];
+  int nk = 0;
+  auto pow2 = [&](int lo, int hi) {
+    std::vector<int> v;
+    for (int x = lo; x <= hi; x *= 2) v.push_back(x);
+    return v;
+  };
+  if (i8) {
+    knobs[nk] = &c->nsk8;
+    for (int x = 8; x <= std::min(128, std::max(8, na)); x += 8) cand[nk].push_back(x);
+    ++nk;
+    knobs[nk] = &c->nsj8;
+    cand[nk++] = pow2(1, std::min(64, na));
+    knobs[nk] = &c->nsg;
+    cand[nk++] = pow2(1, std::min(64, na));
+  } else {
+    knobs[nk] = &c->nsk;
+    cand[nk++] = pow2(1, std::min(128, na));
+    knobs[nk] = &c->nsj;
+    cand[nk++] = pow2(1, std::min(128, na));
+  }
+  float best = 0.f;
+  rc = measure(&best);
+  for (int k = 0; k < nk && !rc; ++k) {
+    int keep = *knobs[k];
+    for (int v : cand[k]) {
+      if (v == keep) continue;
+      *knobs[k] = v;
+      float ms = 0.f;
+      rc = measure(&ms);
+      if (rc) break;
+      if (ms < best) {
+        best = ms;
+        keep = v;
+      }
+    }
+    *knobs[k] = keep;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  if (!rc && best_ms) *best_ms = best;
+  return rc;
 }
 
 int nopt_jk_set_timing(nopt_jk_ctx* c, int enable) {
@@ -1402,10 +1516,8 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
   if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf + 5L * c->splane;
   b += (long)(c->X_bytes + c->Cp_bytes + c->Cd_bytes + c->Ce_bytes + c->Xd_bytes + c->Xe_bytes + c->Jpart_bytes);
-  b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux +
-                         std::max((long)c->nsk * c->ntiles * KT * KT,
-                                  (long)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile) +
-                         kMaxBatch * c->dstride + 5 * c->nbf * c->nbf);
+  b += (long)c->Kpart_bytes;
+  b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux + kMaxBatch * c->dstride + 5 * c->nbf * c->nbf);
   return b;
 }
 

@@ -157,6 +157,28 @@ class JKContext:
                                                    (ctypes.c_long * nd)(*[int(it[2]) for it in items]),
                                                    col(3), col(4), p, stream or None))
 
+    TUNING_KEYS = ("nsk", "nsj", "nsk8", "nsg", "nsj8")
+
+    def tuning(self):
+        """Q-split counts of the build (include/nopt_jk.h, nopt_jk_get_tuning)."""
+        a = np.zeros(5, dtype=np.int32)
+        L.check(L.lib().nopt_jk_get_tuning(self._h, a.ctypes.data_as(L._c_int_p)))
+        return dict(zip(self.TUNING_KEYS, (int(x) for x in a)))
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        t.update(kw)
+        a = np.array([t[k] for k in self.TUNING_KEYS], dtype=np.int32)
+        L.check(L.lib().nopt_jk_set_tuning(self._h, a.ctypes.data_as(L._c_int_p)))
+
+    def autotune(self, dD, dC, nocc, prec="fp64", reps=3, stream=0):
+        """Times the candidate split counts on device buffers (dD, dC: device
+        pointers) and keeps the fastest; returns (tuning, ms per build)."""
+        ms = ctypes.c_double()
+        L.check(L.lib().nopt_jk_autotune(self._h, ctypes.c_void_p(dD), ctypes.c_void_p(dC), int(nocc),
+                                         L.PREC[prec], int(reps), ctypes.byref(ms), ctypes.c_void_p(stream)))
+        return self.tuning(), ms.value
+
     def set_timing(self, enable=True):
         L.check(L.lib().nopt_jk_set_timing(self._h, 1 if enable else 0))
 
