@@ -487,8 +487,10 @@ def main():
     pert = None if args.no_perturbed else perturbed_accuracy(ctx, D, modes, seed=0, count=10, eps=1e-3)
     log(f"[bench] perturbed densities (10, eps 1e-3): {json.dumps(pert)}")
 
-    runs = {}
+    runs, tuning = {}, {}
     for p in modes:
+        # split counts chosen by the autotuner on this density, then the timed builds
+        tuning[p], _ = ctx.autotune(dD.ptr, dC.ptr, nocc, prec=p, reps=3)
         runs[p] = time_builds(ctx, dD, dC, nocc, dJ, dK, p, args.steps, args.warmup, synchronize, timing=True)
         log(f"[bench] {p}: {runs[p][0]:.1f} builds/s ({runs[p][1]:.3f} ms/build); phases {runs[p][2]}")
     if args.prec == "auto":
@@ -567,6 +569,7 @@ def main():
             "modes": {p: {"builds_per_s": round(runs[p][0], 2), "dE_vs_fp64_Eh": acc[p]["dE"],
                           "max_abs_dK": acc[p]["max_abs_dK"],
                           "phase_ms": {k: round(v, 4) for k, v in runs[p][2].items()},
+                          "split_counts": tuning[p],
                           "kernels": {k: (None if v is None else {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                                                   for kk, vv in v.items()})
                                       for k, v in kern[p].items()}} for p in modes},

@@ -827,12 +827,16 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   const long nn = (long)nbf * nbf;
   int rc;
   // integer mode with one column group: the half-transform writes the digit
-  // planes of X directly (per density), X itself is never stored
+  // planes of X directly (per density), X itself is never stored; pairs of
+  // auxiliary functions share the exponent of a row m and lie side by side
+  // (npq rows of pairs), so the exchange streams 2*nocc orbitals per exponent
   const bool xfused = i8 && anyK && ngroups == 1;
+  const long npq = (naux + 1) / 2, rowsq = npq * nbf;
+  auto ldk_of = [&](long nocc) { return xfused ? round_up(2 * nocc, 64) : round_up(nocc, 64); };
   long xoff[kMaxBatch + 1];  // digit-plane offsets (bytes) of each density
   xoff[0] = 0;
   for (int d = 0; d < nd; ++d)
-    xoff[d + 1] = xoff[d] + ((off[d + 1] > off[d]) ? 4 * rows * round_up(it[d].nocc, 64) : 0);
+    xoff[d + 1] = xoff[d] + ((off[d + 1] > off[d]) ? 4 * (xfused ? rowsq : rows) * ldk_of(it[d].nocc) : 0);
   if (anyK) {
     if (!xfused) {
       size_t cap = c->X_bytes;
@@ -853,11 +857,11 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       c->Ce_bytes = b2;
       long ldk_max = 0;
       for (int d = 0; d < nd; ++d)
-        if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, round_up(it[d].nocc, 64));
+        if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, ldk_of(it[d].nocc));
       // fused: planes of every density side by side; otherwise one density
       // at a time
       const size_t xdb = xfused ? (size_t)xoff[nd] : (size_t)4 * rows * ldk_max;
-      const size_t xeb = sizeof(int) * rows * (xfused ? nd : 1);
+      const size_t xeb = sizeof(int) * (xfused ? rowsq * nd : rows);
       size_t b3 = c->Xd_bytes, b4 = c->Xe_bytes;
       rc = ensure(&c->Xd, &b3, xdb, "exchange digit planes");
       if (!rc) rc = ensure(&c->Xe, &b4, xeb, "exchange row exponents");
@@ -977,16 +981,18 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
         for (int d = 0; d < nd; ++d) {
           if (off[d + 1] == off[d]) continue;
           const int k = xs.nseg++;
-          const long ldk = round_up(it[d].nocc, 64);
+          const long ldk = ldk_of(it[d].nocc);
           xs.off[k] = (int)off[d];
           xs.len[k] = (int)it[d].nocc;
           xs.ldk[k] = ldk;
-          xs.xplane[k] = rows * ldk;
+          xs.xplane[k] = rowsq * ldk;
           xs.xd[k] = (int8_t*)c->Xd + xoff[d];
-          xs.xe[k] = (int*)c->Xe + (long)d * rows;
+          xs.xe[k] = (int*)c->Xe + (long)d * rowsq;
         }
-      e = nopt::launch_half_i8(nt, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, (const int8_t*)c->Cd,
-                               ldx * c->ld8, (const int*)c->Ce, xfused ? nullptr : (double*)c->X, ldx, xs);
+      if (xfused) gh.y = (unsigned)npq;
+      e = nopt::launch_half_i8(nt, xfused ? 2 : 1, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux,
+                               (const int8_t*)c->Cd, ldx * c->ld8, (const int*)c->Ce,
+                               xfused ? nullptr : (double*)c->X, ldx, xs);
     }
     else if (f64)
       e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X, ldx,
@@ -1018,13 +1024,16 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     const unsigned nel = (unsigned)((nn + 255) / 256);
     if (i8) {
-      const long ldk = round_up(it[d].nocc, 64);
+      const long ldk = ldk_of(it[d].nocc);
       const int8_t* xd = (const int8_t*)c->Xd + (xfused ? xoff[d] : 0);
-      const int* xe = (const int*)c->Xe + (xfused ? (long)d * rows : 0);
+      const int* xe = (const int*)c->Xe + (xfused ? (long)d * rowsq : 0);
       if (!xfused)
         HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
                                    rows, rows * ldk, (int*)xe, s));
-      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, rows * ldk, xe, ldk, nbf, naux, c->Kpart));
+      // the exchange runs over the rows of the digit planes: pairs of
+      // functions (npq of them) when fused, single functions otherwise
+      const long nq = xfused ? npq : naux;
+      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, nq * nbf * ldk, xe, ldk, nbf, (int)nq, c->Kpart));
       k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
                                      it[d].K);
     } else {

@@ -176,41 +176,58 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 // from LDS per 13*IT*JT products. k-slabs of 64 bytes, LDS double buffered
 // (one barrier per slab) with the next B slab prefetched into registers while
 // the current one is multiplied. NW = 8 halves the accumulators per wave so
-// that twice as many waves hide the memory latency.
-template <int NT, int WC, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_half_i8(
-    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf,
+// that twice as many waves hide the memory latency. QPB = 2 gives a block two
+// auxiliary functions, Q = 2 blockIdx.y and Q + 1 (NW waves each, one shared C
+// slab); with the fused split their rows share one exponent per m and are
+// written side by side, so that the exchange reads 2 x len orbitals per
+// exponent (Q + 1 = naux contributes zeros).
+template <int NT, int WC, int NW, int QPB>
+__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_half_i8(
+    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
-  constexpr int BMI = 64, BN = 16 * NT, KS = 64, NTH = 64 * NW;
+  constexpr int BMI = 64, BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
   constexpr int WR = NW / WC, IT = 4 / WR, JT = NT / WC;
   static_assert(NT % WC == 0 && NW % WC == 0 && IT * WR == 4, "tile split");
-  constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per buffer
-  constexpr int APT = 1024 / NTH;                     // 16-byte B chunks per thread (4 planes x 64 rows x 4)
+  constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per B (one Q) and C buffer
+  constexpr int SB = QPB * AB + CB;                   // bytes per stage
+  constexpr int APT = 1024 * QPB / NTH;               // 16-byte B chunks per thread (QPB x 4 planes x 64 rows x 4)
   constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
-  constexpr int CPT = CCH / NTH;
-  static_assert(CCH % NTH == 0 && 1024 % NTH == 0, "slabs must split evenly over the block");
-  __shared__ __attribute__((aligned(16))) int8_t sm[2][AB + CB];
+  constexpr int CPT = (CCH + NTH - 1) / NTH;
+  static_assert(1024 % (64 * NW) == 0, "B slab must split evenly over the block");
+  __shared__ __attribute__((aligned(16))) int8_t sm[2][SB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wr = w / WC, wc = w % WC;
+  const int qq = w / NW, w8 = w % NW;  // which Q of the block, wave within it
+  const int wr = w8 / WC, wc = w8 % WC;
   const int m0 = blockIdx.x * BMI;
-  const long Q = blockIdx.y;
+  const long Qb = (long)blockIdx.y * QPB;  // first Q of the block
+  const long Q = Qb + qq;                  // this wave's Q
   const int cg = blockIdx.z;
-  // B loader: chunk e = t + NTH u is chunk e%4 of row (e%256)/4 in plane e/256
+  // B loader: chunk e = t + NTH u is chunk e%4 of row (e%256)/4 in plane
+  // (e%1024)/256 of the block's Q number e/1024 (rows beyond nbf and a Q
+  // beyond naux load zeros)
   const int ar = (t & 255) >> 2, ac = t & 3;
-  const bool arow_ok = m0 + ar < nbf;
-  const int8_t* Ap = Bd + (Q * nbf + m0 + ar) * ld8 + 16 * ac;
+  const int8_t* ap[APT];
+  bool aok[APT];
+#pragma unroll
+  for (int u = 0; u < APT; ++u) {
+    const int e = t + NTH * u, qe = e >> 10, pe = (e & 1023) >> 8;
+    aok[u] = m0 + ar < nbf && Qb + qe < naux;
+    ap[u] = Bd + ((Qb + qe) * nbf + m0 + ar) * ld8 + 16 * ac + pe * bplane;
+  }
   const int8_t* Cbase = Cd + (long)cg * BN * ld8;
-  // C loader: chunk e = t + NTH u of the slab is chunk (e & 3) of column
+  // C loader: chunk e = t + NTH u < CCH of the slab is chunk (e & 3) of column
   // (e >> 2) % BN in plane e / (4 BN); global and LDS offsets per u
   int c_g[CPT], c_s[CPT];
 #pragma unroll
   for (int u = 0; u < CPT; ++u) {
-    const int e = t + NTH * u;
+    const int e = min(t + NTH * u, CCH - 1);
     const int c = e & 3, col = (e >> 2) % BN, p = e / (4 * BN);
     c_g[u] = (int)(p * cplane + (long)col * ld8 + 16 * c);
-    c_s[u] = (p * BN + col) * KS + 16 * swz(col, c);
+    c_s[u] = QPB * AB + (p * BN + col) * KS + 16 * swz(col, c);
   }
+  constexpr bool c_even = CCH % NTH == 0;
+  const bool c_last = c_even || t + NTH * (CPT - 1) < CCH;  // this thread has a chunk in the last round
   // prefetch registers; the C chunks sit in two arrays of at most 16 dwords
   // indexed with compile-time constants, so that both stay in registers
   constexpr int CH = (CPT + 1) / 2;
@@ -219,14 +236,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
 #define NOPT_GLOAD_A(k0)                                                                                 \
   do {                                                                                                   \
     _Pragma("unroll") for (int u = 0; u < APT; ++u) a_reg[u] =                                           \
-        arow_ok ? *reinterpret_cast<const uint4*>(Ap + ((t + NTH * u) >> 8) * bplane + (k0))             \
-                : uint4{0, 0, 0, 0};                                                                     \
+        aok[u] ? *reinterpret_cast<const uint4*>(ap[u] + (k0)) : uint4{0, 0, 0, 0};                       \
   } while (0)
 #define NOPT_GLOAD_C(k0)                                                                                 \
   do {                                                                                                   \
     static_for<CPT>([&](auto U) {                                                                        \
       constexpr int u_ = decltype(U)::value;                                                             \
-      const uint4 v_ = *reinterpret_cast<const uint4*>(Cbase + c_g[u_] + (k0));                          \
+      const uint4 v_ = (u_ < CPT - 1 || c_last) ? *reinterpret_cast<const uint4*>(Cbase + c_g[u_] + (k0)) \
+                                                : uint4{0, 0, 0, 0};                                     \
       if constexpr (u_ < CH) c_lo[u_ < CH ? u_ : 0] = v_;                                                \
       else c_hi[u_ < CH ? 0 : u_ - CH] = v_;                                                             \
     });                                                                                                  \
@@ -234,14 +251,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
 #define NOPT_LSTORE(buf)                                                                                 \
   do {                                                                                                   \
     int8_t* As_ = sm[buf];                                                                               \
-    _Pragma("unroll") for (int u = 0; u < APT; ++u)                                                      \
-        *reinterpret_cast<uint4*>(As_ + (((t + NTH * u) >> 8) * BMI + ar) * KS + 16 * swz(ar, ac)) =     \
-            a_reg[u];                                                                                    \
+    _Pragma("unroll") for (int u = 0; u < APT; ++u) {                                                    \
+      const int e_ = t + NTH * u;                                                                        \
+      *reinterpret_cast<uint4*>(As_ + (e_ >> 10) * AB + (((e_ & 1023) >> 8) * BMI + ar) * KS +           \
+                                16 * swz(ar, ac)) = a_reg[u];                                            \
+    }                                                                                                    \
     static_for<CPT>([&](auto U) {                                                                        \
       constexpr int u_ = decltype(U)::value;                                                             \
-      uint4* d_ = reinterpret_cast<uint4*>(As_ + AB + c_s[u_]);                                          \
-      if constexpr (u_ < CH) *d_ = c_lo[u_ < CH ? u_ : 0];                                               \
-      else *d_ = c_hi[u_ < CH ? 0 : u_ - CH];                                                            \
+      uint4* d_ = reinterpret_cast<uint4*>(As_ + c_s[u_]);                                               \
+      if (u_ < CPT - 1 || c_last) {                                                                      \
+        if constexpr (u_ < CH) *d_ = c_lo[u_ < CH ? u_ : 0];                                             \
+        else *d_ = c_hi[u_ < CH ? 0 : u_ - CH];                                                          \
+      }                                                                                                  \
     });                                                                                                  \
   } while (0)
   v4i_t acc[IT][JT][5];
@@ -262,8 +283,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
     // the B slab (HBM) is fetched before the products, the small C slab
     // (L2-resident) after them (measured faster than fetching both first)
     if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
-    const int8_t* As = sm[cur];
-    const int8_t* Cs = sm[cur] + AB;
+    const int8_t* As = sm[cur] + qq * AB;
+    const int8_t* Cs = sm[cur] + QPB * AB;
     uint4 av[IT][4];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
@@ -328,7 +349,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int m = m0 + rowl(i, v);
-      const int em = m < nbf ? Be[Q * nbf + m] : 0;
+      const bool live = m < nbf && Q < naux;
+      const int em = live ? Be[Q * nbf + m] : 0;
 #pragma unroll
       for (int j = 0; j < JT; ++j) {
         // 2^28 x the digit sum: hi = 128 A0 + A1 is exact in int32 (|A0| <= 4096 ld8,
@@ -337,7 +359,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
         const int hi = A[0][v] * 128 + A[1][v];
         const double sv =
             fma((double)hi, 0x1p21, fma((double)A[2][v], 0x1p14, fma((double)A[3][v], 128.0, (double)A[4][v])));
-        xv[i][j][v] = m < nbf ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
+        xv[i][j][v] = live ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
       }
     }
   if (xs.nseg == 0) {
@@ -346,7 +368,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int m = m0 + rowl(i, v);
-        if (m < nbf) {
+        if (m < nbf && Q < naux) {
           double* xr = X + ((long)Q * nbf + m) * ldx + cg * BN;
 #pragma unroll
           for (int j = 0; j < JT; ++j) xr[colb(j)] = xv[i][j][v];
@@ -356,18 +378,22 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
   }
   // fused digit split (one column group): per segment the row exponent (the
   // largest frexp exponent over the segment's columns: 16 lanes of a wave,
-  // then the WC waves of a row via LDS), then each value as the integer
-  // N = rint(x 2^(27-e)), |N| <= 2^27, in balanced base-128 digits
-  // N = 2^21 d0 + 2^14 d1 + 2^7 d2 + d3 (|d| <= 64), staged in LDS and written
-  // out as 16-byte chunks.
-  // The staging tile (at most 4 x 64 x 128 bytes) overlays the operand
-  // buffers; with WC > 1 it fits the first and the row exponents of the WC
-  // waves sit in the second (no LDS beyond the operand buffers, so that two
-  // blocks share a CU)
-  static_assert(4 * BMI * 128 <= (WC > 1 ? 1 : 2) * (AB + CB) && WC * BMI * 4 <= AB + CB, "epilogue LDS");
+  // then the WC waves of a row and the QPB functions of the block via LDS),
+  // then each value as the integer N = rint(x 2^(27-e)), |N| <= 2^27, in
+  // balanced base-128 digits N = 2^21 d0 + 2^14 d1 + 2^7 d2 + d3 (|d| <= 64),
+  // staged in LDS (row blockIdx.y*nbf + m: the block's functions side by
+  // side, len columns each) and written out as 16-byte chunks.
+  // The staging tile overlays the operand buffers: both stages when the row
+  // exponents stay in registers, else the first, with the exponents of the
+  // QPB x WC waves of a row in the second (no LDS beyond the operand
+  // buffers).
+  constexpr bool kRegExp = WC == 1 && QPB == 1;
+  constexpr int kLdkMax = (QPB * BN + 63) / 64 * 64;
+  static_assert(4 * BMI * kLdkMax <= (kRegExp ? 2 : 1) * SB && QPB * WC * BMI * 4 <= SB, "epilogue LDS");
   constexpr int kNoExp = -4096;  // exponent of an all-zero row segment
   int8_t* st = &sm[0][0];
   int(*rexp)[BMI] = reinterpret_cast<int(*)[BMI]>(&sm[1][0]);
+  const long orow = (long)blockIdx.y * nbf;  // first output row of the block
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
     const int ldk = (int)xs.ldk[d];
@@ -389,10 +415,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
         }
 #pragma unroll
         for (int sh = 1; sh < 16; sh <<= 1) ex = max(ex, __shfl_xor(ex, sh, 64));
-        if constexpr (WC == 1)
+        if constexpr (kRegExp)
           rx[i][v] = ex;
         else if (lr == 0)
-          rexp[wc][rowl(i, v)] = ex;
+          rexp[qq * WC + wc][rowl(i, v)] = ex;
       }
     // orders the zeroing of the staging tile (and the row exponents of the
     // other waves) before the digit writes below
@@ -404,19 +430,20 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
         const int row = rowl(i, v);
         const int m = m0 + row;
         int e;
-        if constexpr (WC == 1) {
+        if constexpr (kRegExp) {
           e = rx[i][v];
         } else {
           e = rexp[0][row];
 #pragma unroll
-          for (int q = 1; q < WC; ++q) e = max(e, rexp[q][row]);
+          for (int q = 1; q < QPB * WC; ++q) e = max(e, rexp[q][row]);
         }
         if (e == kNoExp) e = 0;
-        if (wc == 0 && lr == 0 && m < nbf) xs.xe[d][Q * nbf + m] = e;
+        if (qq == 0 && wc == 0 && lr == 0 && m < nbf) xs.xe[d][orow + m] = e;
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
           const int c0 = wc * 16 * JT + j * 16;
-          if (c0 >= o && c0 < o + pad) {
+          const int cl = c0 + lr - o;  // column within the segment
+          if (c0 >= o && c0 < o + pad && cl < len) {
             int n = (int)rint(ldexp(xv[i][j][v], 27 - e));
             const int d3 = ((n + 64) & 127) - 64;
             n = (n - d3) >> 7;
@@ -424,7 +451,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
             n = (n - d2) >> 7;
             const int d1 = ((n + 64) & 127) - 64;
             n = (n - d1) >> 7;
-            int8_t* sp = st + row * ldk + c0 + lr - o;
+            int8_t* sp = st + row * ldk + qq * len + cl;
             sp[0] = (int8_t)n;
             sp[BMI * ldk] = (int8_t)d1;
             sp[2 * BMI * ldk] = (int8_t)d2;
@@ -438,7 +465,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
       const int p = i / (BMI * cpr), rem = i % (BMI * cpr), row = rem / cpr, ch = rem % cpr;
       const int m = m0 + row;
       if (m < nbf)
-        *reinterpret_cast<uint4*>(xs.xd[d] + p * xs.xplane[d] + (Q * nbf + m) * (long)ldk + 16 * ch) =
+        *reinterpret_cast<uint4*>(xs.xd[d] + p * xs.xplane[d] + (orow + m) * (long)ldk + 16 * ch) =
             reinterpret_cast<const uint4*>(st)[i];
     }
     __syncthreads();
@@ -903,21 +930,31 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
   return hipGetLastError();
 }
 
-hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx,
-                          const XSplit& xs) {
-  if (ld8 % 64 != 0 || (long)grid.z * 16 * nt > ldx) return hipErrorInvalidValue;
+hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                          long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
+                          long ldx, const XSplit& xs) {
+  if (ld8 % 64 != 0 || (long)grid.z * 16 * nt > ldx || (qpb != 1 && qpb != 2)) return hipErrorInvalidValue;
+  if ((long)grid.y * qpb < naux || (long)(grid.y - 1) * qpb >= naux) return hipErrorInvalidValue;
   if (xs.nseg < 0 || xs.nseg > 4 || (xs.nseg > 0 && grid.z != 1)) return hipErrorInvalidValue;
+  const long ldk_max = (qpb * 16 * nt + 63) / 64 * 64;
   for (int d = 0; d < xs.nseg; ++d)
-    if (xs.off[d] % 16 || xs.len[d] <= 0 || xs.ldk[d] % 64 || xs.ldk[d] > 128 || xs.ldk[d] < xs.len[d] ||
-        xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] || !xs.xe[d])
+    if (xs.off[d] % 16 || xs.len[d] <= 0 || xs.ldk[d] % 64 || xs.ldk[d] > ldk_max ||
+        xs.ldk[d] < (long)qpb * xs.len[d] || xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] ||
+        !xs.xe[d])
       return hipErrorInvalidValue;
-  switch (nt) {
-    case 2: k_half_i8<2, 1, 4><<<grid, 256, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 4: k_half_i8<4, 2, 8><<<grid, 512, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
-    case 6: k_half_i8<6, 2, 8><<<grid, 512, 0, s>>>(Bd, bplane, Be, ld8, nbf, Cd, cplane, Ce, X, ldx, xs); break;
+#define NOPT_HALF_I8(NT_, WC_, NW_, QPB_)                                                                   \
+  k_half_i8<NT_, WC_, NW_, QPB_><<<grid, 64 * NW_ * QPB_, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Cd, cplane, \
+                                                                  Ce, X, ldx, xs)
+  switch (nt * 4 + qpb) {
+    case 2 * 4 + 1: NOPT_HALF_I8(2, 1, 4, 1); break;
+    case 4 * 4 + 1: NOPT_HALF_I8(4, 2, 8, 1); break;
+    case 6 * 4 + 1: NOPT_HALF_I8(6, 2, 8, 1); break;
+    case 2 * 4 + 2: NOPT_HALF_I8(2, 1, 4, 2); break;
+    case 4 * 4 + 2: NOPT_HALF_I8(4, 2, 8, 2); break;
+    case 6 * 4 + 2: NOPT_HALF_I8(6, 2, 8, 2); break;
     default: return hipErrorInvalidValue;
   }
+#undef NOPT_HALF_I8
   return hipGetLastError();
 }
 

@@ -25,9 +25,9 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
                            int8_t* planes, long ldo, long pstride, hipStream_t s);
 // Digit split of the half-transformed rows fused into the half-transform:
 // segment d covers the columns [off[d], off[d] + len[d]) of X (off a multiple
-// of 16); its rows (Q, m) get their own exponent xe[d][Q*nbf + m] and four
-// digit planes xd[d][p*xplane[d] + (Q*nbf + m)*ldk[d] + (col - off[d])],
-// zero beyond len[d] (ldk[d] a multiple of 64, at most 128).
+// of 16); its rows get their own exponent xe[d][row] and four digit planes
+// xd[d][p*xplane[d] + row*ldk[d] + col], zero beyond the written columns
+// (ldk[d] a multiple of 64; rows and columns as launch_half_i8 states).
 struct XSplit {
   int nseg;
   int off[4], len[4];
@@ -37,13 +37,16 @@ struct XSplit {
 };
 
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
-// (grid: ceil(nbf/64) x naux x column groups of 16*nt columns, nt in {2,4,6};
-// ld8 a multiple of 64).
+// (grid: ceil(nbf/64) x ceil(naux/qpb) x column groups of 16*nt columns, nt in
+// {2,4,6}; ld8 a multiple of 64).
 // With xs.nseg > 0 (one column group only) the rows are split into digit
-// planes in the epilogue and X is not written.
-hipError_t launch_half_i8(int nt, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be, long ld8,
-                          int nbf, const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx,
-                          const XSplit& xs);
+// planes in the epilogue and X is not written. qpb = 2 gives each block the
+// functions 2y and 2y+1, whose rows (y*nbf + m) share one exponent, the
+// function 2y in columns [0, len) and 2y+1 in [len, 2 len) of the segment
+// (ldk >= 2 len); qpb = 1 writes row Q*nbf + m.
+hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
+                          long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
+                          long ldx, const XSplit& xs);
 // Coulomb pair over B's lower triangle in packed rows: row m of B_Q as
 // m/16 + 1 segments of 16 columns (zeros beyond column m), the rows one after
 // the other, five digit planes Bs[k*splane + Q*sq + 16*seg0(m) + n], sq =
