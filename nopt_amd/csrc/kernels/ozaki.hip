@@ -180,21 +180,23 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 // auxiliary functions, Q = 2 blockIdx.y and Q + 1 (NW waves each, one shared C
 // slab); with the fused split their rows share one exponent per m and are
 // written side by side, so that the exchange reads 2 x len orbitals per
-// exponent (Q + 1 = naux contributes zeros).
-template <int NT, int WC, int NW, int QPB>
-__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_half_i8(
+// exponent (Q + 1 = naux contributes zeros). BMI rows per tile (64, or 32 for
+// two functions, so that two blocks still share a CU); WPE waves per SIMD.
+template <int NT, int WC, int NW, int QPB, int BMI, int WPE>
+__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_half_i8(
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
-  constexpr int BMI = 64, BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
-  constexpr int WR = NW / WC, IT = 4 / WR, JT = NT / WC;
-  static_assert(NT % WC == 0 && NW % WC == 0 && IT * WR == 4, "tile split");
+  constexpr int BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
+  constexpr int WR = NW / WC, IT = BMI / (16 * WR), JT = NT / WC;
+  static_assert(NT % WC == 0 && NW % WC == 0 && IT * WR * 16 == BMI, "tile split");
   constexpr int AB = 4 * BMI * KS, CB = 4 * BN * KS;  // bytes per B (one Q) and C buffer
   constexpr int SB = QPB * AB + CB;                   // bytes per stage
-  constexpr int APT = 1024 * QPB / NTH;               // 16-byte B chunks per thread (QPB x 4 planes x 64 rows x 4)
+  constexpr int ACQ = 16 * BMI;                       // 16-byte B chunks per Q (4 planes x BMI rows x 4)
+  constexpr int APT = ACQ * QPB / NTH;                // per thread
   constexpr int CCH = 4 * BN * 4;                     // 16-byte C chunks per slab
   constexpr int CPT = (CCH + NTH - 1) / NTH;
-  static_assert(1024 % (64 * NW) == 0, "B slab must split evenly over the block");
+  static_assert(ACQ * QPB % NTH == 0 && NTH % (4 * BMI) == 0, "B slab must split evenly over the block");
   __shared__ __attribute__((aligned(16))) int8_t sm[2][SB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int qq = w / NW, w8 = w % NW;  // which Q of the block, wave within it
@@ -203,15 +205,15 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(N
   const long Qb = (long)blockIdx.y * QPB;  // first Q of the block
   const long Q = Qb + qq;                  // this wave's Q
   const int cg = blockIdx.z;
-  // B loader: chunk e = t + NTH u is chunk e%4 of row (e%256)/4 in plane
-  // (e%1024)/256 of the block's Q number e/1024 (rows beyond nbf and a Q
-  // beyond naux load zeros)
-  const int ar = (t & 255) >> 2, ac = t & 3;
+  // B loader: chunk e = t + NTH u is chunk e%4 of row (e/4)%BMI in plane
+  // (e%ACQ)/(4 BMI) of the block's Q number e/ACQ (rows beyond nbf and a Q
+  // beyond naux load zeros); the row and chunk do not depend on u
+  const int ar = (t >> 2) % BMI, ac = t & 3;
   const int8_t* ap[APT];
   bool aok[APT];
 #pragma unroll
   for (int u = 0; u < APT; ++u) {
-    const int e = t + NTH * u, qe = e >> 10, pe = (e & 1023) >> 8;
+    const int e = t + NTH * u, qe = e / ACQ, pe = (e % ACQ) / (4 * BMI);
     aok[u] = m0 + ar < nbf && Qb + qe < naux;
     ap[u] = Bd + ((Qb + qe) * nbf + m0 + ar) * ld8 + 16 * ac + pe * bplane;
   }
@@ -253,7 +255,7 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(N
     int8_t* As_ = sm[buf];                                                                               \
     _Pragma("unroll") for (int u = 0; u < APT; ++u) {                                                    \
       const int e_ = t + NTH * u;                                                                        \
-      *reinterpret_cast<uint4*>(As_ + (e_ >> 10) * AB + (((e_ & 1023) >> 8) * BMI + ar) * KS +           \
+      *reinterpret_cast<uint4*>(As_ + (e_ / ACQ) * AB + (((e_ % ACQ) / (4 * BMI)) * BMI + ar) * KS +     \
                                 16 * swz(ar, ac)) = a_reg[u];                                            \
     }                                                                                                    \
     static_for<CPT>([&](auto U) {                                                                        \
@@ -942,16 +944,18 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_
         xs.ldk[d] < (long)qpb * xs.len[d] || xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] ||
         !xs.xe[d])
       return hipErrorInvalidValue;
-#define NOPT_HALF_I8(NT_, WC_, NW_, QPB_)                                                                   \
-  k_half_i8<NT_, WC_, NW_, QPB_><<<grid, 64 * NW_ * QPB_, 0, s>>>(Bd, bplane, Be, ld8, nbf, naux, Cd, cplane, \
-                                                                  Ce, X, ldx, xs)
+#define NOPT_HALF_I8(NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                  \
+  k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z), 64 * NW_ * QPB_, 0, \
+                                               s>>>(Bd, bplane, Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs)
+  // 64-row tiles of one function per block (two blocks per CU), or 32-row
+  // tiles of two functions (two blocks per CU)
   switch (nt * 4 + qpb) {
-    case 2 * 4 + 1: NOPT_HALF_I8(2, 1, 4, 1); break;
-    case 4 * 4 + 1: NOPT_HALF_I8(4, 2, 8, 1); break;
-    case 6 * 4 + 1: NOPT_HALF_I8(6, 2, 8, 1); break;
-    case 2 * 4 + 2: NOPT_HALF_I8(2, 1, 4, 2); break;
-    case 4 * 4 + 2: NOPT_HALF_I8(4, 2, 8, 2); break;
-    case 6 * 4 + 2: NOPT_HALF_I8(6, 2, 8, 2); break;
+    case 2 * 4 + 1: NOPT_HALF_I8(2, 1, 4, 1, 64, 2); break;
+    case 4 * 4 + 1: NOPT_HALF_I8(4, 2, 8, 1, 64, 4); break;
+    case 6 * 4 + 1: NOPT_HALF_I8(6, 2, 8, 1, 64, 4); break;
+    case 2 * 4 + 2: NOPT_HALF_I8(2, 1, 4, 2, 64, 4); break;
+    case 4 * 4 + 2: NOPT_HALF_I8(4, 2, 4, 2, 32, 4); break;
+    case 6 * 4 + 2: NOPT_HALF_I8(6, 2, 4, 2, 32, 4); break;
     default: return hipErrorInvalidValue;
   }
 #undef NOPT_HALF_I8

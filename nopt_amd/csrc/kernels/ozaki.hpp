@@ -37,8 +37,9 @@ struct XSplit {
 };
 
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
-// (grid: ceil(nbf/64) x ceil(naux/qpb) x column groups of 16*nt columns, nt in
-// {2,4,6}; ld8 a multiple of 64).
+// (grid: row tiles (the launcher picks their height) x ceil(naux/qpb) x
+// column groups of 16*nt columns, nt in {2,4,6}; ld8 a multiple of 64; the
+// caller's grid.x is ignored).
 // With xs.nseg > 0 (one column group only) the rows are split into digit
 // planes in the epilogue and X is not written. qpb = 2 gives each block the
 // functions 2y and 2y+1, whose rows (y*nbf + m) share one exponent, the
