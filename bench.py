@@ -70,12 +70,12 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
 # kernels of each timed phase, per mode
 KERNELS = {
     "half_transform": {"fp64": "k_half<double, 6>", "fp32": "k_half<float, 6>", "bf16x6": "k_half_bf16<3, 6>",
-                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 2, 8>"},
+                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 2, 4, 2, 32, 4>"},
     "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
               "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)",
-              "int8": "k_gamma_slots<1, 2>"},
+              "int8": "k_gamma_slots<1, 2, true>"},
     "coulomb_j": {"fp64": "k_jpart<1>", "fp32": "k_jpart<1>", "bf16x6": "k_jpart<1>", "bf16x3": "k_jpart<1>",
-                  "int8": "k_jpart_slots<1, 2>"},
+                  "int8": "k_jpart_slots<1, 2, true>"},
     "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
                  "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8"},
 }

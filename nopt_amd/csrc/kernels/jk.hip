@@ -906,7 +906,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
     } else if (i8) {
       CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                    c->nslots, c->Dp, ldb, c->dstride, c->gslot, c->gamma, naux));
+                                    c->nslots, c->segs, c->nseg, c->Dp, ldb, c->dstride, c->gslot, c->gamma, naux));
     } else {
 #define NOPT_GF(ND)                                                                                         \
   case ND:                                                                                                  \
@@ -919,7 +919,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     cmark(3);
     if (i8)
       return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                      c->nslots, c->gamma, naux, Jpart, (long)nsj * nn);
+                                      c->nslots, c->segs, c->nseg, c->gamma, naux, Jpart, (long)nsj * nn);
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                     \
   case ND:                                                                                              \
@@ -1178,7 +1178,10 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
     c->nsj8 = (int)std::max(1L, std::min((long)naux, (4L * ncu + nbj - 1) / nbj));
     rc = dev_alloc((void**)&c->slots, sizeof(int) * c->nslots, "Coulomb slot table");
     if (!rc) rc = dev_alloc((void**)&c->segs, sizeof(int) * c->nseg, "Coulomb segment table");
-    if (!rc) rc = dev_alloc((void**)&c->gslot, sizeof(double) * nsb * kMaxBatch * naux, "gamma partials (slots)");
+    if (!rc)
+      rc = dev_alloc((void**)&c->gslot,
+                     sizeof(double) * nopt::gamma_partial_blocks(c->nslots, c->nseg) * kMaxBatch * naux,
+                     "gamma partials");
     if (!rc && (hipMemcpy(c->slots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess ||
                 hipMemcpy(c->segs, stab.data(), sizeof(int) * c->nseg, hipMemcpyHostToDevice) != hipSuccess))
       rc = set_error(NOPT_EHIP, "nopt_jk_create: copy of the slot tables failed");

@@ -522,6 +522,15 @@ __device__ __forceinline__ void load_dig8(const int8_t* b, long splane, Dig8& d)
 #pragma unroll
   for (int k = 0; k < 5; ++k) d.p[k] = *reinterpret_cast<const uint2*>(b + k * splane);
 }
+// the two 8-column halves of a 16-column segment, one 16-byte load per plane
+__device__ __forceinline__ void load_dig16(const int8_t* b, long splane, Dig8& lo, Dig8& hi) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const uint4 v = *reinterpret_cast<const uint4*>(b + k * splane);
+    lo.p[k] = uint2{v.x, v.y};
+    hi.p[k] = uint2{v.z, v.w};
+  }
+}
 __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
   const int8_t* dg = reinterpret_cast<const int8_t*>(d.p);
 #pragma unroll
@@ -537,7 +546,7 @@ __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
 // time over all 64*SPL slots of the block (SPL per lane, the weighted density
 // values held in registers) and reduces across its lanes; the digits of its
 // next Q are loaded while the current one is reduced.
-template <int ND, int SPL>
+template <int ND, int SPL, bool SEG>
 __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bs, long splane, long sq,
                                                      const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
@@ -546,14 +555,16 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
   const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  static_assert(!SEG || SPL == 2, "a segment is two slots");
   int mrow[SPL], ncol[SPL];
   double dv[ND][SPL][8];
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
-    const int sl = sb * 64 * SPL + k * 64 + lane;
+    // SEG: the table lists 16-column segments, lane's slots are its two halves
+    const int sl = SEG ? sb * 64 + lane : sb * 64 * SPL + k * 64 + lane;
     const int code = sl < nslots ? slots[sl] : -1;
     mrow[k] = code >= 0 ? code >> 16 : 0;  // row 0 stands in for an empty slot (zero weights)
-    ncol[k] = code >= 0 ? code & 0xffff : 0;
+    ncol[k] = code >= 0 ? (code & 0xffff) + (SEG ? 8 * k : 0) : 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -568,10 +579,15 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
 #pragma unroll
   for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
+    if constexpr (SEG) {
+      load_dig16(Bs + (long)q * sq + boff[0], splane, dst[0], dst[1]);
+      e[0] = e[1] = Be[(long)q * nbf + mrow[0]];
+    } else {
 #pragma unroll
-    for (int k = 0; k < SPL; ++k) {
-      load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
-      e[k] = Be[(long)q * nbf + mrow[k]];
+      for (int k = 0; k < SPL; ++k) {
+        load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+        e[k] = Be[(long)q * nbf + mrow[k]];
+      }
     }
   };
   int q = q0 + w;
@@ -621,7 +637,7 @@ __global__ void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, i
 // each thread owns SPL slots (8 columns each) and accumulates over the Q of
 // its split in registers, loading the digits of the next Q while the current
 // one is accumulated.
-template <int ND, int SPL>
+template <int ND, int SPL, bool SEG>
 __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bs, long splane, long sq,
                                                      const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
@@ -634,11 +650,11 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
   double acc[ND][SPL][8];
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
-    const int sl = sb * 256 * SPL + k * 256 + threadIdx.x;
+    const int sl = SEG ? sb * 256 + (int)threadIdx.x : sb * 256 * SPL + k * 256 + threadIdx.x;
     const int code = sl < nslots ? slots[sl] : -1;
     live[k] = code >= 0;
     mrow[k] = live[k] ? code >> 16 : 0;  // row 0 stands in for an empty slot (result not written)
-    ncol[k] = live[k] ? code & 0xffff : 0;
+    ncol[k] = live[k] ? (code & 0xffff) + (SEG ? 8 * k : 0) : 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
@@ -650,10 +666,15 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
 #pragma unroll
   for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
+    if constexpr (SEG) {
+      load_dig16(Bs + (long)q * sq + boff[0], splane, dst[0], dst[1]);
+      e[0] = e[1] = Be[(long)q * nbf + mrow[0]];
+    } else {
 #pragma unroll
-    for (int k = 0; k < SPL; ++k) {
-      load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
-      e[k] = Be[(long)q * nbf + mrow[k]];
+      for (int k = 0; k < SPL; ++k) {
+        load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+        e[k] = Be[(long)q * nbf + mrow[k]];
+      }
     }
   };
   if (q0 < q1) fetch(q0, cur, ecur);
@@ -993,14 +1014,17 @@ void coulomb_slot_table(int nbf, int* out) {
 int gamma_slot_blocks(int nslots) { return (nslots + 64 * kGammaSPL - 1) / (64 * kGammaSPL); }
 
 hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
-                              int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
-                              long dstride, double* gpart, double* gamma, long gstride) {
-  const dim3 grid(gamma_slot_blocks(nslots), nsplit);
+                              int nbf, int naux, const int* slots, int nslots, const int* segs, int nseg,
+                              const double* Dp, long ldd, long dstride, double* gpart, double* gamma, long gstride) {
+  // one or two densities: a 16-column segment per lane (16-byte loads);
+  // more: 8-column slots
+  const int nsb = nd <= 2 ? (nseg + 63) / 64 : gamma_slot_blocks(nslots);
+  const dim3 grid(nsb, nsplit);
   switch (nd) {
-    case 1: k_gamma_slots<1, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 2: k_gamma_slots<2, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 3: k_gamma_slots<3, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
-    case 4: k_gamma_slots<4, kGammaSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 1: k_gamma_slots<1, 2, true><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, segs, nseg, Dp, ldd, dstride, gpart); break;
+    case 2: k_gamma_slots<2, 2, true><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, segs, nseg, Dp, ldd, dstride, gpart); break;
+    case 3: k_gamma_slots<3, kGammaSPL, false><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
+    case 4: k_gamma_slots<4, kGammaSPL, false><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
     default: return hipErrorInvalidValue;
   }
   const int n = nd * naux;
@@ -1008,15 +1032,16 @@ hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
   return hipGetLastError();
 }
 
+int gamma_partial_blocks(int nslots, int nseg) { return std::max(gamma_slot_blocks(nslots), (nseg + 63) / 64); }
+
 hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
-                              int nbf, int naux, const int* slots, int nslots, const double* gamma,
-                              long gstride, double* Jpart, long jstride) {
-  const dim3 grid((nslots + 256 * kJSPL - 1) / (256 * kJSPL), nsplit);
+                              int nbf, int naux, const int* slots, int nslots, const int* segs, int nseg,
+                              const double* gamma, long gstride, double* Jpart, long jstride) {
   switch (nd) {
-    case 1: k_jpart_slots<1, kJSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 2: k_jpart_slots<2, kJSPL><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 3: k_jpart_slots<3, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
-    case 4: k_jpart_slots<4, 1><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 1: k_jpart_slots<1, 2, true><<<dim3((nseg + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, segs, nseg, gamma, gstride, Jpart, jstride); break;
+    case 2: k_jpart_slots<2, 2, true><<<dim3((nseg + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, segs, nseg, gamma, gstride, Jpart, jstride); break;
+    case 3: k_jpart_slots<3, 1, false><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    case 4: k_jpart_slots<4, 1, false><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

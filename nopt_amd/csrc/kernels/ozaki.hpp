@@ -65,12 +65,15 @@ hipError_t pack_segments(const int8_t* Bd, long bplane, long ld8, int nbf, long 
 int coulomb_slot_count(int nbf);
 void coulomb_slot_table(int nbf, int* out);
 int gamma_slot_blocks(int nslots);
+// One or two densities walk the 16-column segments instead (one 16-byte load
+// per plane and lane); gamma_partial_blocks bounds the partials of either.
 hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
-                              int nbf, int naux, const int* slots, int nslots, const double* Dp, long ldd,
-                              long dstride, double* gpart, double* gamma, long gstride);
+                              int nbf, int naux, const int* slots, int nslots, const int* segs, int nseg,
+                              const double* Dp, long ldd, long dstride, double* gpart, double* gamma, long gstride);
+int gamma_partial_blocks(int nslots, int nseg);
 hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* Bs, long splane, long sq, const int* Be,
-                              int nbf, int naux, const int* slots, int nslots, const double* gamma,
-                              long gstride, double* Jpart, long jstride);
+                              int nbf, int naux, const int* slots, int nslots, const int* segs, int nseg,
+                              const double* gamma, long gstride, double* Jpart, long jstride);
 
 // Exchange partials over 64x64 lower-triangle tiles: Kpart[split][tile] for
 // ntiles tiles and nsplit Q-splits (a multiple of 8, one contiguous range of
