@@ -725,20 +725,26 @@ __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
 // Kpart[s][tile][r][c] = sum_{Q in split s} 2^(e_Qm + e_Qn - 12) sum_{k+l<=4}
 //   2^(-7(k+l)) <Xd_k[Q][m][:], Xd_l[Q][n][:]>,  m = 64 bi + r, n = 64 bj + c,
 // k, l < 4 (13 digit pairs in 5 exact int32 classes, combined in FP64 once
-// per auxiliary function). Tile 64x64: 4 waves of 32x32 (2x2 MFMA blocks);
-// the operands stream as 64-byte k-slabs over (Q, orbital chunk), double
-// buffered in swizzled LDS with one barrier per slab; each thread prefetches
-// the row exponents of its own accumulator rows and columns.
+// per auxiliary function; with the pair rows of the fused half-transform a
+// "function" here is a pair of them). Tile 64x64: NW waves in a (NW/2) x 2
+// grid (NW = 8: 16x32 each, four waves per SIMD); the operands stream as
+// 64-byte k-slabs over (Q, orbital chunk), double buffered in swizzled LDS
+// with one barrier per slab; each thread loads the row exponents of its own
+// accumulator rows and columns at the start of a function's last slab.
 constexpr int KT8 = 64;
 //
 // Blocks are numbered so that each XCD (workgroups go to the 8 XCDs round
 // robin) owns a contiguous range of Q-splits with all their tiles: the tiles
 // of one split read the same X rows at about the same time, from that XCD's
 // L2. The grid is 1-D, ntiles * nsplit blocks, nsplit a multiple of 8.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_syrk_i8(
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_syrk_i8(
     const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
     int ntiles, int nsplit, double* __restrict__ Kpart) {
+  // NW waves in a (NW/2) x 2 grid: each wave 64/(NW/2) rows (IT row tiles) x 32 columns
   constexpr int KS = 64, TB = 4 * KT8 * KS;  // bytes of one operand tile (4 planes)
+  constexpr int NTH = 64 * NW, IT = 64 / (NW / 2) / 16, OPT = 512 / NTH;  // operands loaded per thread
+  static_assert(IT >= 1 && OPT >= 1 && IT * 16 * (NW / 2) == 64, "tile split");
   __shared__ __attribute__((aligned(16))) int8_t sm[2][2 * TB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
   const int lr = lane & 15, lc = lane >> 4;
@@ -750,41 +756,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   const int nkq = (int)(ldk / KS);  // k-slabs per auxiliary function
   const int nsl = (q1 - q0) * nkq;
-  // loader: row t/4 of each operand tile, chunk t%4, all 4 planes
-  const int ar = t >> 2, ac = t & 3;
-  const int ma = bi * KT8 + ar, mb = bj * KT8 + ar;
-  const bool ra = ma < nbf, rb = mb < nbf;
-  const int8_t* pa = Xd + (long)ma * ldk + 16 * ac;
-  const int8_t* pb = Xd + (long)mb * ldk + 16 * ac;
+  // loader: row (t/4)%64 of operand tile (t/256 + o) (o < OPT; tile 0 rows
+  // bi, tile 1 rows bj), chunk t%4, all 4 planes
+  const int ar = (t >> 2) & 63, ac = t & 3, op0 = (OPT == 2) ? 0 : (t >> 8);
+  const int8_t* pr[OPT];
+  bool rok[OPT];
+#pragma unroll
+  for (int o = 0; o < OPT; ++o) {
+    const int m = ((op0 + o) == 0 ? bi : bj) * KT8 + ar;
+    rok[o] = m < nbf;
+    pr[o] = Xd + (long)m * ldk + 16 * ac;
+  }
   const long qstride = (long)nbf * ldk;
-  uint4 a_reg[4], b_reg[4];
+  uint4 o_reg[OPT][4];
   auto gload = [&](int sl) {
     const int q = q0 + sl / nkq, k0 = (sl % nkq) * KS;
     const long off = (long)q * qstride + k0;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      a_reg[p] = ra ? *reinterpret_cast<const uint4*>(pa + p * xplane + off) : uint4{0, 0, 0, 0};
-      b_reg[p] = rb ? *reinterpret_cast<const uint4*>(pb + p * xplane + off) : uint4{0, 0, 0, 0};
-    }
+    for (int o = 0; o < OPT; ++o)
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+        o_reg[o][p] = rok[o] ? *reinterpret_cast<const uint4*>(pr[o] + p * xplane + off) : uint4{0, 0, 0, 0};
   };
   auto lstore = [&](int buf) {
-    int8_t* As = sm[buf];
-    int8_t* Bs = sm[buf] + TB;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-      *reinterpret_cast<uint4*>(As + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = a_reg[p];
-      *reinterpret_cast<uint4*>(Bs + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = b_reg[p];
+    for (int o = 0; o < OPT; ++o) {
+      int8_t* Ts = sm[buf] + (op0 + o) * TB;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(Ts + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = o_reg[o][p];
     }
   };
-  // exponents of this thread's accumulator rows (2 x 4) and columns (2)
-  int ea[2][4], eb[2], na[2][4], nb[2];
-  auto eload = [&](int q, int (&xa)[2][4], int (&xb)[2]) {
+  // exponents of this thread's accumulator rows (IT x 4) and columns (2)
+  int ea[IT][4], eb[2];
+  auto eload = [&](int q, int (&xa)[IT][4], int (&xb)[2]) {
     const int* e = Xe + (long)q * nbf;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int m = bi * KT8 + wr * 32 + i * 16 + 4 * lc + v;
+        const int m = bi * KT8 + wr * 16 * IT + i * 16 + 4 * lc + v;
         xa[i][v] = m < nbf ? e[m] : 0;
       }
 #pragma unroll
@@ -793,10 +803,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       xb[j] = n < nbf ? e[n] : 0;
     }
   };
-  v4i_t acc[2][2][5];
-  d4_t k64[2][2];
+  v4i_t acc[IT][2][5];
+  d4_t k64[IT][2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       k64[i][j] = d4_t{0, 0, 0, 0};
@@ -805,7 +815,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
   if (nsl > 0) {
     gload(0);
-    eload(q0, ea, eb);
     lstore(0);
   }
   __syncthreads();
@@ -814,13 +823,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const bool more = sl + 1 < nsl;
     const bool qend = (sl % nkq) == nkq - 1;
     if (more) gload(sl + 1);
-    if (more && qend) eload(q0 + (sl + 1) / nkq, na, nb);
+    // the exponents of this function's rows, needed after its last slab
+    if (qend) eload(q0 + sl / nkq, ea, eb);
     const int8_t* As = sm[cur];
     const int8_t* Bs = sm[cur] + TB;
-    uint4 av[2][4];
+    uint4 av[IT][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int r = wr * 32 + i * 16 + lr;
+    for (int i = 0; i < IT; ++i) {
+      const int r = wr * 16 * IT + i * 16 + lr;
 #pragma unroll
       for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * KT8 + r) * KS + 16 * swz(r, lc));
     }
@@ -830,7 +840,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       const int8_t* cp = Bs + cidx * KS + 16 * swz(cidx, lc);
       uint4 bv = *reinterpret_cast<const uint4*>(cp);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IT; ++i) {
         v4i_t* A = acc[i][j];
         A[0] = mfma_i8(av[i][0], bv, A[0]);
         A[1] = mfma_i8(av[i][1], bv, A[1]);
@@ -839,7 +849,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
       bv = *reinterpret_cast<const uint4*>(cp + KT8 * KS);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IT; ++i) {
         v4i_t* A = acc[i][j];
         A[1] = mfma_i8(av[i][0], bv, A[1]);
         A[2] = mfma_i8(av[i][1], bv, A[2]);
@@ -848,7 +858,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
       bv = *reinterpret_cast<const uint4*>(cp + 2 * KT8 * KS);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IT; ++i) {
         v4i_t* A = acc[i][j];
         A[2] = mfma_i8(av[i][0], bv, A[2]);
         A[3] = mfma_i8(av[i][1], bv, A[3]);
@@ -856,7 +866,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       }
       bv = *reinterpret_cast<const uint4*>(cp + 3 * KT8 * KS);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < IT; ++i) {
         v4i_t* A = acc[i][j];
         A[3] = mfma_i8(av[i][0], bv, A[3]);
         A[4] = mfma_i8(av[i][1], bv, A[4]);
@@ -867,7 +877,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       // and ldk <= 1024, hi = 128 A0 + A1 and mid = 128 A2 + A3 are exact in
       // int32 and 2^21 hi + 2^7 mid + A4 (= 2^28 x the digit sum) in FP64
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < IT; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           v4i_t* A = acc[i][j];
@@ -880,26 +890,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
           for (int c = 0; c < 5; ++c) A[c] = v4i_t{0, 0, 0, 0};
         }
-      if (more) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) ea[i][v] = na[i][v];
-        eb[0] = nb[0];
-        eb[1] = nb[1];
-      }
+
     }
     if (more) lstore(cur ^ 1);
     __syncthreads();
   }
   double* Kt = Kpart + ((long)sp * ntiles + tile) * KT8 * KT8;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int rr = wr * 32 + i * 16 + 4 * lc + v;
+        const int rr = wr * 16 * IT + i * 16 + 4 * lc + v;
         const int cc = wc * 32 + j * 16 + lr;
         Kt[rr * KT8 + cc] = k64[i][j][v];
       }
@@ -1050,7 +1053,7 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart) {
   if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
-  k_syrk_i8<<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
+  k_syrk_i8<8><<<(unsigned)(ntiles * nsplit), 512, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
   return hipGetLastError();
 }
 
