@@ -893,9 +893,9 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
   };
   // Coulomb pair (gamma, J partials) and the J output. Without timing they
-  // run on the side stream, forked after the staging (or after the
-  // half-transform when gamma rides on it) and joined before the build ends,
-  // so that they overlap the exchange.
+  // run on the side stream, forked after the staging (after the
+  // half-transform in the integer mode, or when gamma rides on it) and joined
+  // before the build ends, so that they overlap the exchange.
   const bool overlap = !tm && nj > 0 && anyK;
   hipStream_t sj = overlap ? c->side : s;
   auto cmark = [&](int ph) {
@@ -964,7 +964,10 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       stage_rows<float>(c->Dp, ldb, 0, 0, (float*)c->Cp + ncol * ldb, ldb, tail, s);
   }
   HIPCHK(hipGetLastError());
-  if (overlap && !fused) {
+  // the integer Coulomb pair starts after the half-transform (it then shares
+  // the device with the exchange; started earlier it takes CUs from the
+  // half-transform, measured slower)
+  if (overlap && !fused && !i8) {
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     HIPCHK(coulomb());
@@ -1006,7 +1009,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     HIPCHK(e);
   }
   mark(2);
-  if (overlap && fused) {
+  if (overlap && (fused || i8)) {
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     HIPCHK(coulomb());
