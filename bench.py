@@ -77,7 +77,7 @@ KERNELS = {
     "coulomb_j": {"fp64": "k_jpart<1>", "fp32": "k_jpart<1>", "bf16x6": "k_jpart<1>", "bf16x3": "k_jpart<1>",
                   "int8": "k_jpart_slots<1, 2, true>"},
     "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
-                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8"},
+                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8<8>"},
 }
 
 
@@ -250,12 +250,22 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
         res[p] = dict(eager_ms=1e3 * eager, graph_ms=1e3 * graph, builds_per_s_graph=2 * count / graph,
                       builds_per_s_eager=2 * count / eager, graph_vs_eager_max_abs_diff=same,
                       graph_ms_default_splits=untuned[p], outputs=ref)
-    # accuracy of the mixed mode against FP64 over the whole batch
+    # accuracy of the mixed mode against FP64 over the whole batch, as the
+    # two-electron energy of each density: sum Dc.(2J - K) for the inactive
+    # density (unit occupations, closed shell) and 1/2 sum Dt.J - 1/4 sum Dt.K
+    # for the spin-summed total density (occupations 0..2), which is a
+    # quarter of sum Dt.(2J - K)
     dE = dK = 0.0
-    for d, r64, rp in zip(systems, res["fp64"]["outputs"], res[prec]["outputs"]):
+    worst = {}
+    for d, n, r64, rp in zip(systems, sizes, res["fp64"]["outputs"], res[prec]["outputs"]):
         Dc, Cc, Dt, Ct = d["host"]
-        for (D, J64, K64, J, K) in ((Dc, r64[0], r64[1], rp[0], rp[1]), (Dt, r64[2], r64[3], rp[2], rp[3])):
-            dE = max(dE, abs(energy_terms(D, J, K) - energy_terms(D, J64, K64)))
+        for kind, w, D, J64, K64, J, K in (("inactive", 1.0, Dc, r64[0], r64[1], rp[0], rp[1]),
+                                           ("total", 0.25, Dt, r64[2], r64[3], rp[2], rp[3])):
+            e64 = w * energy_terms(D, J64, K64)
+            de = abs(w * energy_terms(D, J, K) - e64)
+            if de > worst.get(kind, {}).get("dE", -1.0):
+                worst[kind] = dict(dE=de, E2=e64, waters=int(n))
+            dE = max(dE, de)
             dK = max(dK, float(np.abs(K - K64).max()))
     for p in res:
         res[p].pop("outputs")
@@ -266,7 +276,7 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
     return dict(clusters=count, waters=int(sum(sizes)), sizes=sizes, builds_per_graph=2 * count, autotune_s=round(tune_s, 1),
                 nbf_range=[min(d["m"].nbf for d in systems), max(d["m"].nbf for d in systems)],
                 setup_s=round(setup, 1), modes={p: {k: round(v, 4) for k, v in r.items()} for p, r in res.items()},
-                max_abs_dE_vs_fp64=dE, max_abs_dK_vs_fp64=dK, precision=prec)
+                max_abs_dE_vs_fp64=dE, max_abs_dK_vs_fp64=dK, worst_dE_by_density=worst, precision=prec)
 
 
 def large_system(n_water, seed, steps, threads):
@@ -367,9 +377,11 @@ def large_system(n_water, seed, steps, threads):
     }
 
 
-def cpu_baseline(sym, xyz_ang, threads, nbuilds=3, label="C20H42"):
+def cpu_baseline(sym, xyz_ang, threads, nbuilds=5, label="C20H42"):
     """NOPT's CPU path (oracle restatement) on a bounded sample: one warm-up
-    and `nbuilds` timed builds of the full C20H42 Cartesian workload."""
+    and `nbuilds` timed builds of the full C20H42 Cartesian workload (V^-1/2
+    applied per build, as DM_to_F_transform_RI does), then the same for the
+    "B-prepared" variant, whose stored tensor already holds the metric."""
     os.environ["ORACLE_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -381,23 +393,37 @@ def cpu_baseline(sym, xyz_ang, threads, nbuilds=3, label="C20H42"):
     RI_AA = mc.eri3_nopt()
     Vm05, _, _ = O.s05(mc.metric())
     O.lib().oracle_set_threads(threads)
-    jk = O.NoptJK(RI_AA, Vm05, mc.nocc)
     rng = np.random.default_rng(0)
     MO, _ = np.linalg.qr(rng.standard_normal((mc.nbf, mc.nbf)))
     MO = np.ascontiguousarray(MO.T)
     DM = MO[:mc.nocc].T @ MO[:mc.nocc]
     setup = time.perf_counter() - t0
-    jk(DM, MO, mc.nocc)
-    ts = []
-    for _ in range(nbuilds):
-        t1 = time.perf_counter()
+
+    def median_s(jk):
         jk(DM, MO, mc.nocc)
-        ts.append(time.perf_counter() - t1)
-    med = float(np.median(ts))
+        ts = []
+        for _ in range(nbuilds):
+            t1 = time.perf_counter()
+            jk(DM, MO, mc.nocc)
+            ts.append(time.perf_counter() - t1)
+        return float(np.median(ts))
+
+    jk = O.NoptJK(RI_AA, Vm05, mc.nocc)
+    med = median_s(jk)
+    del jk
+    t1 = time.perf_counter()
+    jkp = O.NoptJKPrepared(RI_AA, Vm05, mc.nocc)
+    prep_s = time.perf_counter() - t1
+    del RI_AA
+    medp = median_s(jkp)
+    del jkp
     return dict(value=1.0 / med, unit="builds/s", cores=threads, kind="port",
                 sample=f"{nbuilds} timed builds after 1 warm-up of the oracle's C restatement of NOPT's "
                        f"DM_to_F_transform_RI (MKL DGEMM, OpenMP row split as nopt_par_dgemm) on {label}: "
                        f"Cartesian n_ao {mc.nbf}, naux {mc.naux}, n_c {mc.nocc}; median {med:.3f} s/build",
+                b_prepared=dict(value=1.0 / medp, unit="builds/s", median_s=round(medp, 4), prepare_s=round(prep_s, 1),
+                                sample=f"{nbuilds} timed builds after 1 warm-up, same workload with RI_AA.V^-1/2 "
+                                       f"formed once (no metric products per build)"),
                 setup_s=round(setup, 1))
 
 
@@ -529,7 +555,7 @@ def main():
     if not args.no_ragged:
         ragged = ragged_batch(prec, max(3, args.steps // 5), 1)
         log(f"[bench] ragged batch ({ragged['clusters']} clusters, {ragged['builds_per_graph']} builds per graph): "
-            f"{json.dumps(ragged['modes'])}")
+            f"{json.dumps(ragged['modes'])}; worst |dE| vs fp64 {json.dumps(ragged['worst_dE_by_density'])}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -538,8 +564,8 @@ def main():
             a.free()
         t0 = time.perf_counter()
         cpu = cpu_baseline(sym, xyz, args.cpu_threads, label=f"C{args.alkane}H{2 * args.alkane + 2}")
-        log(f"[bench] CPU baseline {cpu['value']:.3f} builds/s on {cpu['cores']} threads "
-            f"({time.perf_counter() - t0:.1f} s)")
+        log(f"[bench] CPU baseline {cpu['value']:.3f} builds/s on {cpu['cores']} threads, B-prepared variant "
+            f"{cpu['b_prepared']['value']:.3f} builds/s ({time.perf_counter() - t0:.1f} s)")
 
     line = {
         "metric": "RI-JK Fock builds/s (C20H42 cc-pVDZ), mixed vs FP64 GPU at |dE|<1 uEh",

@@ -4,8 +4,9 @@
 // e_r = exponent of the row's largest magnitude, and the scaled row
 // v = x 2^-e_r (|v| < 1) is written as signed base-128 digits
 //   x 2^(6-e_r) = d_0 + 2^-7 d_1 + 2^-14 d_2 + ... ,  d_k = rint(...) in [-64, 64],
-// each digit rounded to nearest, so the truncation error is unbiased and
-// below 2^-(6+7S) of the row maximum for S digits. A product of two split
+// each digit rounded to nearest, so the truncation error is unbiased and at
+// most 2^(e_r - 7S), i.e. 2^(1-7S) of the row maximum for S digits (2^-27
+// for 4 digits, 2^-34 for 5). A product of two split
 // operands is the weighted sum of digit-plane products; every digit-plane
 // product is an exact int32 dot product on v_mfma_i32_16x16x64_i8, so the
 // accumulation adds no rounding at all. Digit pairs of equal weight (k+l)

@@ -63,6 +63,11 @@ def lib():
         L.oracle_jk_nopt.restype = ctypes.c_int
         L.oracle_jk_nopt.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_long, ctypes.c_long,
                                      _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_prepare_b.restype = None
+        L.oracle_prepare_b.argtypes = [_dp, _dp, _dp, ctypes.c_long, ctypes.c_long]
+        L.oracle_jk_nopt_prepared.restype = ctypes.c_int
+        L.oracle_jk_nopt_prepared.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                              _dp, _dp, _dp]
         _lib = L
         L.oracle_set_threads(int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1)))
     return _lib
@@ -278,6 +283,32 @@ class NoptJK:
         MO = np.ascontiguousarray(MO, dtype=np.float64)
         lib().oracle_jk_nopt(_d(J), _d(K), _d(DM), _d(MO), n_c, n, self.naux, _d(self.RI_AA), _d(self.Vm05),
                              _d(self.RI_CA), _d(self.RI_CA2), _d(self.RI_J), _d(self.RI_J2))
+        return J, K
+
+
+class NoptJKPrepared:
+    """The same build over RI_B = RI_AA.V^-1/2, formed once per geometry (no
+    V^-1/2 products per build: src/RI.cpp:120-171 without :128-138 and
+    :152-156)."""
+
+    def __init__(self, RI_AA, Vm05, n_c):
+        RI_AA = np.ascontiguousarray(RI_AA)
+        self.n_ao, self.naux = RI_AA.shape[0], RI_AA.shape[2]
+        self.RI_B = np.empty_like(RI_AA)
+        lib().oracle_prepare_b(_d(self.RI_B), _d(RI_AA), _d(np.ascontiguousarray(Vm05)), self.n_ao, self.naux)
+        self.n_c = n_c
+        self.RI_CA = np.zeros(2 * n_c * self.n_ao * self.naux)
+        self.RI_J = np.zeros(self.naux)
+
+    def __call__(self, DM, MO, n_c):
+        if n_c > self.n_c:
+            raise ValueError("n_c exceeds the scratch size")
+        n = self.n_ao
+        J = np.zeros((n, n))
+        K = np.zeros((n, n))
+        lib().oracle_jk_nopt_prepared(_d(J), _d(K), _d(np.ascontiguousarray(DM, dtype=np.float64)),
+                                      _d(np.ascontiguousarray(MO, dtype=np.float64)), n_c, n, self.naux,
+                                      _d(self.RI_B), _d(self.RI_CA), _d(self.RI_J))
         return J, K
 
 

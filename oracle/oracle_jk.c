@@ -15,6 +15,12 @@
  *                       half-transform MO.RI_AA, the metric product with
  *                       V^-1/2, the [i][m][Q] -> [m][i][Q] transpose and the
  *                       full (not symmetric-rank-k) K product
+ *   oracle_jk_nopt_prepared  the same build with the metric folded into the
+ *                       stored tensor once, RI_B = RI_AA.V^-1/2 (the CPU
+ *                       "B-prepared" variant): gamma = DM.RI_B, J = RI_B.gamma,
+ *                       the half-transform MO.RI_B, the transpose and the full
+ *                       K product, i.e. src/RI.cpp:120-171 without the V^-1/2
+ *                       products of :128-138 and :152-156
  *   oracle_dm           gen_HF_DM (src/SCF.cpp:63-73)
  * BLAS/LAPACK: Intel MKL through libmkl_rt (NOPT supports MKL, Makefile:22-26),
  * with prototypes declared here because no CBLAS/LAPACKE headers are present.
@@ -141,5 +147,32 @@ int oracle_jk_nopt(double* J, double* K, const double* DM, const double* MO, lon
       memcpy(RI_CA + (b * n_c + a) * naux, RI_CA2 + (a * n_ao + b) * naux, sizeof(double) * naux);
   oracle_par_dgemm(NoTrans, Trans, (int)n_ao, (int)n_ao, (int)(naux * n_c), 1.0, RI_CA,
                    (int)(naux * n_c), RI_CA, (int)(naux * n_c), 0.0, K, (int)n_ao);
+  return 1;
+}
+
+/* RI_B = RI_AA.V^-1/2 ([(m*n_ao+n)*naux+P], one row-split GEMM, once per
+ * geometry) and the build over it: J = RI_B (RI_B^T DM), K = Y Y^T with
+ * Y[m][i][P] = sum_n MO[i][n] RI_B[(n*n_ao+m)*naux+P] (RI_B is symmetric in
+ * m, n). RI_CA: scratch of 2*n_c*n_ao*naux doubles; RI_J: naux doubles. */
+void oracle_prepare_b(double* RI_B, const double* RI_AA, const double* Vm05, long n_ao, long naux) {
+  oracle_par_dgemm(NoTrans, NoTrans, (int)(n_ao * n_ao), (int)naux, (int)naux, 1.0, RI_AA, (int)naux, Vm05,
+                   (int)naux, 0.0, RI_B, (int)naux);
+}
+
+int oracle_jk_nopt_prepared(double* J, double* K, const double* DM, const double* MO, long n_c, long n_ao,
+                            long naux, const double* RI_B, double* RI_CA, double* RI_J) {
+  double* RI_CT = RI_CA + n_c * n_ao * naux;
+  memset(RI_J, 0, sizeof(double) * naux);
+  cblas_dgemm(RowMajor, NoTrans, NoTrans, 1, (int)naux, (int)(n_ao * n_ao), 1.0, DM, (int)(n_ao * n_ao), RI_B,
+              (int)naux, 0.0, RI_J, (int)naux);
+  oracle_par_dgemm(NoTrans, NoTrans, (int)(n_ao * n_ao), 1, (int)naux, 1.0, RI_B, (int)naux, RI_J, 1, 0.0, J, 1);
+  oracle_par_dgemm(NoTrans, NoTrans, (int)n_c, (int)(naux * n_ao), (int)n_ao, 1.0, MO, (int)n_ao, RI_B,
+                   (int)(naux * n_ao), 0.0, RI_CA, (int)(naux * n_ao));
+#pragma omp parallel for num_threads(g_threads)
+  for (long b = 0; b < n_ao; ++b)
+    for (long a = 0; a < n_c; ++a)
+      memcpy(RI_CT + (b * n_c + a) * naux, RI_CA + (a * n_ao + b) * naux, sizeof(double) * naux);
+  oracle_par_dgemm(NoTrans, Trans, (int)n_ao, (int)n_ao, (int)(naux * n_c), 1.0, RI_CT, (int)(naux * n_c), RI_CT,
+                   (int)(naux * n_c), 0.0, K, (int)n_ao);
   return 1;
 }

@@ -164,3 +164,19 @@ def test_jk_invariants():
     J2, K2 = r["jk"](D, (C @ U).T, m.nocc)  # rotation among occupied orbitals
     assert np.abs(K2 - K).max() < 1e-11 and np.abs(J2 - J).max() < 1e-12
     assert np.sum(D * J) > 0 and np.sum(D * K) > 0
+
+
+def test_prepared_b_variant_matches_nopt_chain_and_brute_force():
+    """The CPU "B-prepared" build (metric folded into the stored tensor once)
+    gives the J and K of NOPT's chain and of the explicit contraction."""
+    sym, xyz = G.water()
+    m = O.OracleMolecule.from_angstrom(sym, xyz, gbs("cc-pvdz"), gbs("cc-pvdz-jkfit"))
+    r = O.rhf(m)
+    eri3 = m.eri3_nopt()
+    Vm05, _, _ = O.s05(m.metric())
+    jp = O.NoptJKPrepared(eri3, Vm05, m.nocc)
+    J, K = jp(r["D"], r["C"].T, m.nocc)
+    J0, K0 = r["jk"](r["D"], r["C"].T, m.nocc)
+    Jb, Kb = O.brute_force_jk(eri3, m.metric(), r["D"])
+    assert np.abs(J - J0).max() < 1e-11 and np.abs(K - K0).max() < 1e-11
+    assert np.abs(J - Jb).max() < 1e-11 and np.abs(K - Kb).max() < 1e-11
