@@ -145,8 +145,8 @@ int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const dou
  * 8), integer gamma splits, integer J splits}. create picks them from the
  * device's CU count; nopt_jk_autotune times `reps` device builds of (dD, dC)
  * in mode `prec` on `stream` for each candidate of each split count in turn
- * (coordinate descent), keeps the fastest and returns its mean milliseconds
- * per build in *best_ms (may be NULL). Results differ from the untuned build
+ * (two rounds of coordinate descent), keeps the fastest and returns its mean
+ * milliseconds per build in *best_ms (may be NULL). Results differ from the untuned build
  * only in the order of the FP64 partial sums. No counterpart in NOPT, whose
  * DGEMMs split rows evenly over threads (include/blas_link.h:54-116). */
 int nopt_jk_get_tuning(const nopt_jk_ctx* ctx, int* params);

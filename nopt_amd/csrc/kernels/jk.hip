@@ -526,7 +526,7 @@ struct nopt_jk_ctx {
   // second stream for the bandwidth-bound Coulomb pair, which runs beside the
   // MFMA-bound exchange; fork/join events
   hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_half = nullptr, ev_join = nullptr;
   double* B64 = nullptr;
   int flags = 0;        // NOPT_CTX_*: which forms of B the context keeps
   bool b_ready = false; // B formed
@@ -901,7 +901,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   auto cmark = [&](int ph) {
     if (!overlap) mark(ph);
   };
-  auto coulomb = [&]() -> hipError_t {
+  auto coulomb_gamma = [&]() -> hipError_t {
     if (fused) {
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
     } else if (i8) {
@@ -917,6 +917,9 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     CHK_(hipGetLastError());
     cmark(3);
+    return hipSuccess;
+  };
+  auto coulomb_j = [&]() -> hipError_t {
     if (i8)
       return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
                                       c->nslots, c->segs, c->nseg, c->gamma, naux, Jpart, (long)nsj * nn);
@@ -928,6 +931,10 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     switch (nj) { NOPT_JP(1) NOPT_JP(2) NOPT_JP(3) NOPT_JP(4) }
 #undef NOPT_JP
     return hipGetLastError();
+  };
+  auto coulomb = [&]() -> hipError_t {
+    CHK_(coulomb_gamma());
+    return coulomb_j();
   };
   auto coulomb_out = [&]() -> hipError_t {
     for (int k = 0; k < nj; ++k)
@@ -965,8 +972,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   }
   HIPCHK(hipGetLastError());
   // the integer Coulomb pair starts after the half-transform (it then shares
-  // the device with the exchange; started earlier it takes CUs from the
-  // half-transform, measured slower)
+  // the device with the exchange; started earlier, whole or gamma alone, it
+  // takes CUs from the half-transform, measured slower)
   if (overlap && !fused && !i8) {
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
@@ -1010,8 +1017,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   }
   mark(2);
   if (overlap && (fused || i8)) {
-    HIPCHK(hipEventRecord(c->ev_fork, s));
-    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    HIPCHK(hipEventRecord(c->ev_half, s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_half, 0));
     HIPCHK(coulomb());
     HIPCHK(coulomb_out());
     coulomb_at = 1;
@@ -1142,6 +1149,7 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_half, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
@@ -1238,6 +1246,7 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   for (auto& ev : c->tev) (void)hipEventDestroy(ev);
   if (c->side) (void)hipStreamSynchronize(c->side);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_half) (void)hipEventDestroy(c->ev_half);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1323,23 +1332,27 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
     knobs[nk] = &c->nsj;
     cand[nk++] = pow2(1, std::min(128, na));
   }
+  // two rounds: the split counts interact (the Coulomb pair shares the
+  // device with the exchange), so the first knob is tried again once the
+  // others have moved
   float best = 0.f;
   rc = measure(&best);
-  for (int k = 0; k < nk && !rc; ++k) {
-    int keep = *knobs[k];
-    for (int v : cand[k]) {
-      if (v == keep) continue;
-      *knobs[k] = v;
-      float ms = 0.f;
-      rc = measure(&ms);
-      if (rc) break;
-      if (ms < best) {
-        best = ms;
-        keep = v;
+  for (int round = 0; round < 2 && !rc; ++round)
+    for (int k = 0; k < nk && !rc; ++k) {
+      int keep = *knobs[k];
+      for (int v : cand[k]) {
+        if (v == keep) continue;
+        *knobs[k] = v;
+        float ms = 0.f;
+        rc = measure(&ms);
+        if (rc) break;
+        if (ms < best) {
+          best = ms;
+          keep = v;
+        }
       }
+      *knobs[k] = keep;
     }
-    *knobs[k] = keep;
-  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = timing;

@@ -623,15 +623,32 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
   }
 }
 
-// gamma[d][Q] = sum_sb gpart[sb][d][Q]
-__global__ void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, int naux, double* __restrict__ gamma,
-                            long gstride) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nd * naux) return;
-  const int d = i / naux, q = i % naux;
+// gamma[d][Q] = sum_sb gpart[sb][d][Q]. A block takes 64 consecutive Q of one
+// density with four waves: wave p adds the slot blocks sb = p, p+4, p+8, ...
+// (eight loads in flight at a time), then the four sums are added in the
+// order p = 0, 1, 2, 3, so that the result is reproducible.
+__global__ __launch_bounds__(256) void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, int naux,
+                                                   double* __restrict__ gamma, long gstride) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, p = threadIdx.x >> 6;
+  const int d = blockIdx.y, q = blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int b = 0; b < nsb; ++b) s += gpart[((long)b * nd + d) * naux + q];
-  gamma[d * gstride + q] = s;
+  if (q < naux) {
+    const double* g = gpart + (long)d * naux + q;
+    const long st = (long)nd * naux;  // between consecutive slot blocks
+    int b = p;
+    for (; b + 28 < nsb; b += 32) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = g[(long)(b + 4 * u) * st];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; b < nsb; b += 4) s += g[(long)b * st];
+  }
+  part[p][lane] = s;
+  __syncthreads();
+  if (p == 0 && q < naux) gamma[d * gstride + q] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 // Jpart[d][split][m][n] = sum_{Q in split} gamma_d[Q] B_Q,mn over the slots:
@@ -1031,8 +1048,7 @@ hipError_t launch_gamma_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
     case 4: k_gamma_slots<4, kGammaSPL, false><<<grid, 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart); break;
     default: return hipErrorInvalidValue;
   }
-  const int n = nd * naux;
-  k_gamma_sum<<<(n + 255) / 256, 256, 0, s>>>(gpart, (int)grid.x, nd, naux, gamma, gstride);
+  k_gamma_sum<<<dim3((naux + 63) / 64, nd), 256, 0, s>>>(gpart, (int)grid.x, nd, naux, gamma, gstride);
   return hipGetLastError();
 }
 
