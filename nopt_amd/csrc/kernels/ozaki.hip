@@ -385,22 +385,25 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   // then each value as the integer N = rint(x 2^(27-e)), |N| <= 2^27, in
   // balanced base-128 digits N = 2^21 d0 + 2^14 d1 + 2^7 d2 + d3 (|d| <= 64),
   // staged in LDS (row blockIdx.y*nbf + m: the block's functions side by
-  // side, len columns each) and written out as 16-byte chunks.
+  // side, len columns each) as one dword per column holding the four digits
+  // (byte k = digit k; rows of ldk + 4 dwords, so that the four row groups
+  // of a wave write to different banks), and written out per 16 columns:
+  // four 4 x 4 byte transposes give the 16 bytes of each digit plane.
   // The staging tile overlays the operand buffers: both stages when the row
   // exponents stay in registers, else the first, with the exponents of the
   // QPB x WC waves of a row in the second (no LDS beyond the operand
   // buffers).
   constexpr bool kRegExp = WC == 1 && QPB == 1;
   constexpr int kLdkMax = (QPB * BN + 63) / 64 * 64;
-  static_assert(4 * BMI * kLdkMax <= (kRegExp ? 2 : 1) * SB && QPB * WC * BMI * 4 <= SB, "epilogue LDS");
+  static_assert(4 * BMI * (kLdkMax + 4) <= (kRegExp ? 2 : 1) * SB && QPB * WC * BMI * 4 <= SB, "epilogue LDS");
   constexpr int kNoExp = -4096;  // exponent of an all-zero row segment
-  int8_t* st = &sm[0][0];
+  uint32_t* st = reinterpret_cast<uint32_t*>(&sm[0][0]);
   int(*rexp)[BMI] = reinterpret_cast<int(*)[BMI]>(&sm[1][0]);
   const long orow = (long)blockIdx.y * nbf;  // first output row of the block
   for (int d = 0; d < xs.nseg; ++d) {
     const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
-    const int ldk = (int)xs.ldk[d];
-    const int nch = 4 * BMI * ldk / 16;  // 16-byte chunks of the staging tile
+    const int ldk = (int)xs.ldk[d], srow = ldk + 4;
+    const int nch = BMI * srow / 4;  // 16-byte chunks of the staging tile
     for (int i = t; i < nch; i += NTH) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
     int rx[IT][4];
 #pragma unroll
@@ -454,22 +457,36 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
             n = (n - d2) >> 7;
             const int d1 = ((n + 64) & 127) - 64;
             n = (n - d1) >> 7;
-            int8_t* sp = st + row * ldk + qq * len + cl;
-            sp[0] = (int8_t)n;
-            sp[BMI * ldk] = (int8_t)d1;
-            sp[2 * BMI * ldk] = (int8_t)d2;
-            sp[3 * BMI * ldk] = (int8_t)d3;
+            st[row * srow + qq * len + cl] = ((uint32_t)n & 255u) | (((uint32_t)d1 & 255u) << 8) |
+                                             (((uint32_t)d2 & 255u) << 16) | ((uint32_t)d3 << 24);
           }
         }
       }
     __syncthreads();
-    const int cpr = ldk / 16;  // chunks per row
-    for (int i = t; i < nch; i += NTH) {
-      const int p = i / (BMI * cpr), rem = i % (BMI * cpr), row = rem / cpr, ch = rem % cpr;
+    const int cpr = ldk / 16;  // 16-column units per row
+    for (int i = t; i < BMI * cpr; i += NTH) {
+      const int row = i / cpr, ch = i % cpr;
       const int m = m0 + row;
-      if (m < nbf)
+      if (m >= nbf) continue;
+      const uint4* src = reinterpret_cast<const uint4*>(st + row * srow + 16 * ch);
+      uint32_t pl[4][4];  // pl[plane][a]: the plane's bytes of columns 16 ch + 4a .. 4a+3
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const uint4 q = src[a];
+        // byte k of __builtin_amdgcn_perm(x, y, sel) is byte sel_k of y (0..3) or x (4..7)
+        const uint32_t lo01 = __builtin_amdgcn_perm(q.y, q.x, 0x05010400u);  // x0 y0 x1 y1
+        const uint32_t hi01 = __builtin_amdgcn_perm(q.y, q.x, 0x07030602u);  // x2 y2 x3 y3
+        const uint32_t lo23 = __builtin_amdgcn_perm(q.w, q.z, 0x05010400u);  // z0 w0 z1 w1
+        const uint32_t hi23 = __builtin_amdgcn_perm(q.w, q.z, 0x07030602u);  // z2 w2 z3 w3
+        pl[0][a] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);          // x0 y0 z0 w0
+        pl[1][a] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);          // x1 y1 z1 w1
+        pl[2][a] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);          // x2 y2 z2 w2
+        pl[3][a] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);          // x3 y3 z3 w3
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
         *reinterpret_cast<uint4*>(xs.xd[d] + p * xs.xplane[d] + (orow + m) * (long)ldk + 16 * ch) =
-            reinterpret_cast<const uint4*>(st)[i];
+            uint4{pl[p][0], pl[p][1], pl[p][2], pl[p][3]};
     }
     __syncthreads();
   }
