@@ -347,24 +347,34 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   auto rowl = [&](int i, int v) { return wr * 16 * IT + i * 16 + 4 * lc + v; };
   auto colb = [&](int j) { return wc * 16 * JT + j * 16 + lr; };
   double xv[IT][JT][4];
+  // 2^28 x the digit sum: hi = 128 A0 + A1 is exact in int32 (|A0| <= 4096 ld8,
+  // ld8 <= 2048), so is mid = 128 A2 + A3 for ld8 <= 1024 (|A2| <= 3 * 4096 ld8),
+  // and every partial sum below 2^52 is exact in FP64
+  auto combine = [&](auto MID) {
 #pragma unroll
-  for (int i = 0; i < IT; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int m = m0 + rowl(i, v);
-      const bool live = m < nbf && Q < naux;
-      const int em = live ? Be[Q * nbf + m] : 0;
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + rowl(i, v);
+        const bool live = m < nbf && Q < naux;
+        const int em = live ? Be[Q * nbf + m] : 0;
 #pragma unroll
-      for (int j = 0; j < JT; ++j) {
-        // 2^28 x the digit sum: hi = 128 A0 + A1 is exact in int32 (|A0| <= 4096 ld8,
-        // ld8 <= 2048) and every partial sum below 2^52 is exact in FP64
-        const v4i_t* A = acc[i][j];
-        const int hi = A[0][v] * 128 + A[1][v];
-        const double sv =
-            fma((double)hi, 0x1p21, fma((double)A[2][v], 0x1p14, fma((double)A[3][v], 128.0, (double)A[4][v])));
-        xv[i][j][v] = live ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
+        for (int j = 0; j < JT; ++j) {
+          const v4i_t* A = acc[i][j];
+          const int hi = A[0][v] * 128 + A[1][v];
+          double sv;
+          if constexpr (decltype(MID)::value)
+            sv = fma((double)hi, 0x1p21, fma((double)(A[2][v] * 128 + A[3][v]), 128.0, (double)A[4][v]));
+          else
+            sv = fma((double)hi, 0x1p21, fma((double)A[2][v], 0x1p14, fma((double)A[3][v], 128.0, (double)A[4][v])));
+          xv[i][j][v] = live ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
+        }
       }
-    }
+  };
+  if (ld8 <= 1024)
+    combine(std::true_type{});
+  else
+    combine(std::false_type{});
   if (xs.nseg == 0) {
 #pragma unroll
     for (int i = 0; i < IT; ++i)
@@ -450,15 +460,15 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
           const int c0 = wc * 16 * JT + j * 16;
           const int cl = c0 + lr - o;  // column within the segment
           if (c0 >= o && c0 < o + pad && cl < len) {
-            int n = (int)rint(ldexp(xv[i][j][v], 27 - e));
-            const int d3 = ((n + 64) & 127) - 64;
-            n = (n - d3) >> 7;
-            const int d2 = ((n + 64) & 127) - 64;
-            n = (n - d2) >> 7;
-            const int d1 = ((n + 64) & 127) - 64;
-            n = (n - d1) >> 7;
-            st[row * srow + qq * len + cl] = ((uint32_t)n & 255u) | (((uint32_t)d1 & 255u) << 8) |
-                                             (((uint32_t)d2 & 255u) << 16) | ((uint32_t)d3 << 24);
+            // the balanced digits of N at once: M = N + 64 (1 + 2^7 + 2^14)
+            // holds d3 + 64, d2 + 64, d1 + 64 as unsigned 7-bit fields at
+            // bits 0, 7, 14 and d0 above bit 21; each field u becomes the
+            // byte u - 64 (u ^ 64, bit 6 copied into bit 7)
+            const int M = (int)rint(ldexp(xv[i][j][v], 27 - e)) + 1056832;
+            const uint32_t u = (((uint32_t)M >> 14 & 127u) << 8) | (((uint32_t)M >> 7 & 127u) << 16) |
+                               (((uint32_t)M & 127u) << 24);
+            const uint32_t f = u ^ 0x40404000u;
+            st[row * srow + qq * len + cl] = f | ((f & 0x40404000u) << 1) | ((uint32_t)(M >> 21) & 255u);
           }
         }
       }
