@@ -508,15 +508,20 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
 // 16, with the digits of row m (exponent Be[Q*nbf + m]) and zeros beyond
 // column m. Row m holds m/16 + 1 segments, starting at segment
 // seg0(m) = sum_{r<m} (r/16 + 1), and the segments of all rows follow each
-// other, so one B_Q is a contiguous run of sq = 16*nseg bytes per plane:
-// Bs[k*splane + Q*sq + 16*s + j]. seg_code[s] = (m << 16) | n0.
+// other, so one B_Q is a contiguous run of sq = 16*nseg values.
+// seg_code[s] = (m << 16) | n0. A value is stored as the 35-bit integer
+// N = sum_k d_k 2^(7(4-k)) of its five digits (B = N 2^(e-34)), split into
+// an int32 part N >> 8 and an unsigned byte N & 255: the int32 parts at
+// element Q*sq + 16*s + j of Bs read as int32 (the first 4*splane bytes), the
+// bytes at Bs[4*splane + Q*sq + 16*s + j]. A kernel rebuilds N exactly as
+// fma(part, 256, byte), two conversions instead of five digit extractions.
 __host__ __device__ __forceinline__ long seg0(int m) {
   const long a = m / 16, b = m % 16;
   return m + 8 * a * (a - 1) + a * b;
 }
 
-// Bs from the full digit rows: segment s of B_Q copied out of row (Q, m),
-// bytes beyond column m cleared. Grid: segments / 256 x naux.
+// Bs from the full digit rows: segment s of B_Q formed from row (Q, m),
+// values beyond column m cleared. Grid: segments / 256 x naux.
 __global__ __launch_bounds__(256) void k_pack_segments(const int8_t* __restrict__ Bd, long bplane, long ld8, int nbf,
                                                        const int* __restrict__ seg_code, int nseg,
                                                        int8_t* __restrict__ Bs, long splane) {
@@ -526,47 +531,60 @@ __global__ __launch_bounds__(256) void k_pack_segments(const int8_t* __restrict_
   const int code = seg_code[s], m = code >> 16, n0 = code & 0xffff;
   const int valid = m - n0 + 1;  // columns of the row in this segment (1 .. 16)
   const int8_t* src = Bd + (Q * nbf + m) * ld8 + n0;
-  int8_t* dst = Bs + Q * 16L * nseg + 16L * s;
-  for (int k = 0; k < 5; ++k) {
-    uint4 v = *reinterpret_cast<const uint4*>(src + k * bplane);
-    if (valid < 16) {
-      int8_t* b = reinterpret_cast<int8_t*>(&v);
+  uint4 dg[5];
 #pragma unroll
-      for (int j = 0; j < 16; ++j)
-        if (j >= valid) b[j] = 0;
-    }
-    *reinterpret_cast<uint4*>(dst + k * splane) = v;
+  for (int k = 0; k < 5; ++k) dg[k] = *reinterpret_cast<const uint4*>(src + k * bplane);
+  const int8_t* d = reinterpret_cast<const int8_t*>(dg);
+  int part[16];
+  uint32_t low[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    long n = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) n = n * 128 + d[16 * k + j];
+    if (j >= valid) n = 0;
+    part[j] = (int)(n >> 8);
+    low[j / 4] |= (uint32_t)(n & 255) << (8 * (j % 4));
   }
+  const long e = Q * 16L * nseg + 16L * s;  // element of the first column
+  uint4* p32 = reinterpret_cast<uint4*>(reinterpret_cast<int*>(Bs) + e);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    p32[c] = uint4{(uint32_t)part[4 * c], (uint32_t)part[4 * c + 1], (uint32_t)part[4 * c + 2], (uint32_t)part[4 * c + 3]};
+  *reinterpret_cast<uint4*>(Bs + 4 * splane + e) = uint4{low[0], low[1], low[2], low[3]};
 }
 
 // The lower triangle of every B_Q is cut into 8-column slots (m, n0), n0 <= m,
-// listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the five
-// digit planes of a slot as 8-byte loads and rebuild the values exactly:
-//   B 2^(6-e) = 2^-7 hi + 2^-28 lo,  hi = 128 d0 + d1,  lo = 2^14 d2 + 128 d3 + d4.
+// listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the int32
+// parts and the bytes of a slot (two 16-byte and one 8-byte load) and
+// rebuild the integers N = 256 part + byte exactly in FP64.
 struct Dig8 {
-  uint2 p[5];  // 8 digits of each of the five planes
+  uint4 p[2];  // int32 parts of the 8 columns
+  uint2 b;     // their low bytes
 };
-__device__ __forceinline__ void load_dig8(const int8_t* b, long splane, Dig8& d) {
-#pragma unroll
-  for (int k = 0; k < 5; ++k) d.p[k] = *reinterpret_cast<const uint2*>(b + k * splane);
+// element e = Q*sq + 16*seg0(m) + n0 of the packed layout
+__device__ __forceinline__ void load_dig8(const int8_t* Bs, long splane, long e, Dig8& d) {
+  const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const int*>(Bs) + e);
+  d.p[0] = p[0];
+  d.p[1] = p[1];
+  d.b = *reinterpret_cast<const uint2*>(Bs + 4 * splane + e);
 }
-// the two 8-column halves of a 16-column segment, one 16-byte load per plane
-__device__ __forceinline__ void load_dig16(const int8_t* b, long splane, Dig8& lo, Dig8& hi) {
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const uint4 v = *reinterpret_cast<const uint4*>(b + k * splane);
-    lo.p[k] = uint2{v.x, v.y};
-    hi.p[k] = uint2{v.z, v.w};
-  }
+// the two 8-column halves of a 16-column segment
+__device__ __forceinline__ void load_dig16(const int8_t* Bs, long splane, long e, Dig8& lo, Dig8& hi) {
+  const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const int*>(Bs) + e);
+  lo.p[0] = p[0];
+  lo.p[1] = p[1];
+  hi.p[0] = p[2];
+  hi.p[1] = p[3];
+  const uint4 b = *reinterpret_cast<const uint4*>(Bs + 4 * splane + e);
+  lo.b = uint2{b.x, b.y};
+  hi.b = uint2{b.z, b.w};
 }
 __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
-  const int8_t* dg = reinterpret_cast<const int8_t*>(d.p);
+  const int* q = reinterpret_cast<const int*>(d.p);
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(&d.b);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int hi = 128 * (int)dg[j] + (int)dg[8 + j];
-    const int lo = 16384 * (int)dg[16 + j] + 128 * (int)dg[24 + j] + (int)dg[32 + j];
-    v[j] = 0x1p-7 * (double)hi + 0x1p-28 * (double)lo;
-  }
+  for (int j = 0; j < 8; ++j) v[j] = fma((double)q[j], 256.0, (double)b[j]);
 }
 
 // gpart[sb][d][Q] = sum over the slots of slot block sb of B_Q,mn D_d,mn
@@ -603,17 +621,17 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
   }
   Dig8 cur[SPL], nxt[SPL];
   int ecur[SPL], enxt[SPL];
-  int boff[SPL];  // byte offset of the slot in a packed B_Q
+  int boff[SPL];  // element offset of the slot in a packed B_Q
 #pragma unroll
   for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
     if constexpr (SEG) {
-      load_dig16(Bs + (long)q * sq + boff[0], splane, dst[0], dst[1]);
+      load_dig16(Bs, splane, (long)q * sq + boff[0], dst[0], dst[1]);
       e[0] = e[1] = Be[(long)q * nbf + mrow[0]];
     } else {
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+        load_dig8(Bs, splane, (long)q * sq + boff[k], dst[k]);
         e[k] = Be[(long)q * nbf + mrow[k]];
       }
     }
@@ -634,7 +652,7 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
         double sacc = 0.0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) sacc += bv[j] * dv[d][k][j];
-        acc[d] += ldexp(sacc, ecur[k] - 6);
+        acc[d] += ldexp(sacc, ecur[k] - 34);
       }
     }
 #pragma unroll
@@ -707,17 +725,17 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
   }
   Dig8 cur[SPL], nxt[SPL];
   int ecur[SPL], enxt[SPL];
-  int boff[SPL];  // byte offset of the slot in a packed B_Q
+  int boff[SPL];  // element offset of the slot in a packed B_Q
 #pragma unroll
   for (int k = 0; k < SPL; ++k) boff[k] = (int)(16 * seg0(mrow[k])) + ncol[k];
   auto fetch = [&](int q, Dig8 (&dst)[SPL], int (&e)[SPL]) {
     if constexpr (SEG) {
-      load_dig16(Bs + (long)q * sq + boff[0], splane, dst[0], dst[1]);
+      load_dig16(Bs, splane, (long)q * sq + boff[0], dst[0], dst[1]);
       e[0] = e[1] = Be[(long)q * nbf + mrow[0]];
     } else {
 #pragma unroll
       for (int k = 0; k < SPL; ++k) {
-        load_dig8(Bs + (long)q * sq + boff[k], splane, dst[k]);
+        load_dig8(Bs, splane, (long)q * sq + boff[k], dst[k]);
         e[k] = Be[(long)q * nbf + mrow[k]];
       }
     }
@@ -734,7 +752,7 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
       unpack8(cur[k], bv);
 #pragma unroll
       for (int d = 0; d < ND; ++d) {
-        const double gs = ldexp(g[d], ecur[k] - 6);
+        const double gs = ldexp(g[d], ecur[k] - 34);
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[d][k][j] += gs * bv[j];
       }

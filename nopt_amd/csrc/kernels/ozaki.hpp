@@ -50,9 +50,10 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_
                           long ldx, const XSplit& xs);
 // Coulomb pair over B's lower triangle in packed rows: row m of B_Q as
 // m/16 + 1 segments of 16 columns (zeros beyond column m), the rows one after
-// the other, five digit planes Bs[k*splane + Q*sq + 16*seg0(m) + n], sq =
-// 16*nseg (pack_segments copies them out of the full digit rows; seg_code[s]
-// = m << 16 | n0 for each segment). The kernels walk 8-column slots (m, n0)
+// the other, element e = Q*sq + 16*seg0(m) + n, sq = 16*nseg; each value as
+// the 35-bit integer N of its five digits, ((int*)Bs)[e] = N >> 8 and
+// Bs[4*splane + e] = N & 255 (pack_segments forms them from the full digit
+// rows; seg_code[s] = m << 16 | n0 for each segment; Bs holds 5*splane bytes). The kernels walk 8-column slots (m, n0)
 // of the triangle (slot table, m/8 + 1 slots per row): gamma (partials per
 // slot block, then summed) and J partials per Q-split
 // (Jpart[d*jstride + split*nbf*nbf + m*nbf + n], n <= m).
