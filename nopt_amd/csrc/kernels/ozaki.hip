@@ -587,6 +587,31 @@ __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
   for (int j = 0; j < 8; ++j) v[j] = fma((double)q[j], 256.0, (double)b[j]);
 }
 
+// Sum over the 64 lanes of a wave, valid in every lane: rotations within each
+// row of 16 lanes (DPP row_ror 8, 4, 2, 1: register moves, no LDS round
+// trip), then the four row sums read from lanes 0, 16, 32, 48 and added in
+// that order (a fixed order: reproducible).
+template <int N>
+__device__ __forceinline__ double row_ror(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x120 + N, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x120 + N, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (unsigned long long)(unsigned)lo | ((unsigned long long)(unsigned)hi << 32));
+}
+__device__ __forceinline__ double lane_read(double x, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __builtin_bit_cast(double, (unsigned long long)lo | ((unsigned long long)hi << 32));
+}
+__device__ __forceinline__ double wave_sum_rows(double v) {
+  v += row_ror<8>(v);
+  v += row_ror<4>(v);
+  v += row_ror<2>(v);
+  v += row_ror<1>(v);
+  return ((lane_read(v, 0) + lane_read(v, 16)) + lane_read(v, 32)) + lane_read(v, 48);
+}
+
 // gpart[sb][d][Q] = sum over the slots of slot block sb of B_Q,mn D_d,mn
 // (off-diagonal weight 2), Q in split blockIdx.y. Each wave takes one Q at a
 // time over all 64*SPL slots of the block (SPL per lane, the weighted density
@@ -657,7 +682,7 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
     }
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-      const double tot = warp_sum(acc[d]);
+      const double tot = wave_sum_rows(acc[d]);
       if (lane == 0) gpart[((long)sb * ND + d) * naux + q] = tot;
     }
 #pragma unroll
