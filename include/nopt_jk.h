@@ -52,9 +52,11 @@ int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out);
 void nopt_jk_destroy(nopt_jk_ctx* ctx);
 
 /* Which forms of B a context keeps. The default keeps both: FP64 rows (read
- * by the FP64, FP32 and bf16 modes) and five int8 digit planes (read by the
- * integer mode). A large system keeps one of them: FP64 only (8 bytes per
- * element, full rows) or digit planes only (5 bytes per element); a build in
+ * by the FP64, FP32 and bf16 modes) and the integer forms (read by the
+ * integer mode: five int8 digit planes of the full rows, 5 bytes per element,
+ * and the packed lower triangle for the Coulomb pair, 5 bytes per triangle
+ * element). A large system keeps one of them: FP64 only (8 bytes per
+ * element, full rows) or the integer forms only (7.5 bytes per element); a build in
  * a mode whose form is not kept fails with NOPT_ESTATE. NOPT has a single
  * FP64 form of RI_AA (src/RI.cpp:16-32); this choice has no counterpart. */
 #define NOPT_CTX_DEFAULT 0
