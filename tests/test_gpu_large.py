@@ -97,3 +97,41 @@ def test_adjoint_200_rotations(c20):
     assert worst_e["int8"] < 1e-6
     for p in precs:
         assert worst_g[p] < tol[p], p
+
+
+def test_int8_batch_beyond_one_column_group(c20):
+    """Two densities stacked (162 orbital columns): the integer half-transform
+    takes several column groups, writes X in FP64 and splits it per density
+    afterwards (the path of systems with more than 96 occupied orbitals);
+    each density's J and K stay within the bound of its own FP64 build."""
+    m, ctx, D, Co = c20
+    C2 = np.ascontiguousarray(Co[::-1] * 0.8)
+    D2 = C2.T @ C2
+    out = ctx.build_batch([(D, Co, m.nocc), (D2, C2, m.nocc)], prec="int8")
+    for (J, K), (Dd, Cd) in zip(out, ((D, Co), (D2, C2))):
+        J64, K64 = ctx.build(Dd, Cd, m.nocc, prec="fp64")
+        dE = np.sum(Dd * (2 * (J - J64) - (K - K64)))
+        say("batch of two, int8 vs fp64: max|dJ|", np.abs(J - J64).max(), "max|dK|", np.abs(K - K64).max(), "dE", dE)
+        assert np.abs(J - J64).max() < 1e-8 * max(1.0, np.abs(J64).max())
+        assert np.abs(K - K64).max() < 1e-7 * max(1.0, np.abs(K64).max())
+        assert abs(dE) < 1e-6
+
+
+def test_int8_four_column_tiles():
+    """C8H18 (33 occupied orbitals): the integer half-transform with four
+    16-column tiles (33..64 orbital columns, pair rows) against FP64."""
+    m = Molecule.from_angstrom(*G.alkane(8))
+    assert 32 < m.nocc <= 64
+    with JKContext(m.nbf, m.naux) as ctx:
+        ctx.set_B(m.eri3_packed(), m.metric())
+        rng = np.random.default_rng(3)
+        C, _ = np.linalg.qr(rng.standard_normal((m.nbf, m.nbf)))
+        Co = np.ascontiguousarray(C[:, :m.nocc].T)
+        D = Co.T @ Co
+        J64, K64 = ctx.build(D, Co, m.nocc, prec="fp64")
+        J, K = ctx.build(D, Co, m.nocc, prec="int8")
+        dE = np.sum(D * (2 * (J - J64) - (K - K64)))
+        say("C8H18 int8 vs fp64: max|dJ|", np.abs(J - J64).max(), "max|dK|", np.abs(K - K64).max(), "dE", dE)
+        assert np.abs(J - J64).max() < 1e-8 * max(1.0, np.abs(J64).max())
+        assert np.abs(K - K64).max() < 1e-7 * max(1.0, np.abs(K64).max())
+        assert abs(dE) < 1e-6
