@@ -1047,7 +1047,9 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
 hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
                           long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
                           long ldx, const XSplit& xs) {
-  if (ld8 % 64 != 0 || (long)grid.z * 16 * nt > ldx || (qpb != 1 && qpb != 2)) return hipErrorInvalidValue;
+  // ld8 <= 2048 keeps 128 A0 + A1 exact in int32 (the epilogue's class combine)
+  if (ld8 % 64 != 0 || ld8 > 2048 || (long)grid.z * 16 * nt > ldx || (qpb != 1 && qpb != 2))
+    return hipErrorInvalidValue;
   if ((long)grid.y * qpb < naux || (long)(grid.y - 1) * qpb >= naux) return hipErrorInvalidValue;
   if (xs.nseg < 0 || xs.nseg > 4 || (xs.nseg > 0 && grid.z != 1)) return hipErrorInvalidValue;
   const long ldk_max = (qpb * 16 * nt + 63) / 64 * 64;

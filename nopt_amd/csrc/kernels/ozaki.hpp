@@ -38,7 +38,8 @@ struct XSplit {
 
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
 // (grid: row tiles (the launcher picks their height) x ceil(naux/qpb) x
-// column groups of 16*nt columns, nt in {2,4,6}; ld8 a multiple of 64; the
+// column groups of 16*nt columns, nt in {2,4,6}; ld8 a multiple of 64, at
+// most 2048 (nbf <= 2048: the int32 class combine stays exact); the
 // caller's grid.x is ignored).
 // With xs.nseg > 0 (one column group only) the rows are split into digit
 // planes in the epilogue and X is not written. qpb = 2 gives each block the
