@@ -1,7 +1,8 @@
 """NOPT's binary orbital files (.orb, src/binary_files.cpp:248-379): the
 reader and writer, and NOPT's own RHF orbitals of p-benzoquinone
 (examples/dmrg/pBQ/pBQ_2B2_RHF.orb, written by NOPT's RI-RHF of
-pBQ_2A_RHF.inp in GAMESS order and normalisation) carried into this
+pBQ_2A_RHF.inp in GAMESS order and normalisation) and of octatetraene
+(examples/dmrg/4ene/4ene_RHF.orb) carried into this
 package's Cartesian AO order: orthonormal in this package's overlap, and
 their orbital energies, diagonal of the Fock matrix the oracle builds from
 them, are the energies NOPT stored in the file."""
@@ -99,3 +100,32 @@ def test_nopt_orbital_energies_from_oracle_fock(pbq):
     say("max|eps - eps_file|", np.abs(eps - f.eps).max(), "E(file orbitals)", E, "oracle RHF", E_scf)
     assert np.abs(eps - f.eps).max() < 2e-6
     assert abs(E - E_scf) < 1e-9
+
+
+def test_nopt_octatetraene_orbitals():
+    """NOPT's RHF orbitals of octatetraene (examples/dmrg/4ene/4ene_RHF.orb,
+    from 4ene_RHF.inp: cc-pVDZ / cc-pVDZ-JKFIT, D5=1, 162 spherical orbitals
+    over 170 Cartesian AOs): orthonormal after the reordering, their energies
+    are the diagonal of the oracle's NOPT-path Fock matrix at them (NOPT
+    converged to grad 1e-8, 4ene_RHF.inp:29), and its occupied-virtual block
+    vanishes to the same order."""
+    from systems import ENE4, ENE4_ORB
+    m = Molecule.from_angstrom(*ENE4, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
+    f = read_orb(ENE4_ORB)
+    assert (f.n_mo, f.n_ao, m.nbf, m.nocc) == (162, 170, 170, 29) and f.C_beta is None
+    C = gamess_to_libint(f.C, m.function_l())
+    om = O.OracleMolecule.from_angstrom(ENE4[0], ENE4[1], os.path.join(BASIS, "cc-pvdz.gbs"),
+                                        os.path.join(BASIS, "cc-pvdz-jkfit.gbs"))
+    S, T, V = om.one_electron()
+    dev = np.abs(C @ S @ C.T - np.eye(f.n_mo)).max()
+    H = T + V
+    Co = C[:om.nocc]
+    D = Co.T @ Co
+    J, K = O.NoptJK(om.eri3_nopt(), O.s05(om.metric())[0], om.nocc)(D, Co, om.nocc)
+    F = H + 2 * J - K
+    eps = np.einsum("im,mn,in->i", C, F, C)
+    Fai = np.abs(C[:om.nocc] @ F @ C[om.nocc:].T).max()
+    say("max|C S C^T - I|", dev, "max|eps - eps_file|", np.abs(eps - f.eps).max(), "max|F_ai|", Fai)
+    assert dev < 1e-10
+    assert np.abs(eps - f.eps).max() < 5e-6
+    assert Fai < 1e-6
