@@ -1,7 +1,8 @@
 """Edge shapes of the integer build: an odd number of auxiliary functions
 (NH3 / cc-pVDZ-JKFIT, naux 139: the last pair of functions has one member),
 a batch of four densities (one with no occupied orbitals, one J-only, one
-K-only) against single builds and the oracle, and a single occupied orbital."""
+K-only) against single builds and the oracle, a single occupied orbital, and
+the smallest shapes (H2) in every precision mode."""
 import os
 import sys
 
@@ -75,3 +76,27 @@ def test_single_occupied_orbital(nh3, prec):
     tol = (1e-10 if prec == "fp64" else 5e-8) * max(1.0, np.abs(J0).max())
     say("one orbital", prec, "max|dJ|", np.abs(J - J0).max(), "max|dK|", np.abs(K - K0).max())
     assert np.abs(J - J0).max() < tol and np.abs(K - K0).max() < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("int8", 5e-8), ("fp32", 5e-6), ("bf16x6", 5e-6),
+                                      ("bf16x3", 2e-4)])
+def test_smallest_shapes(prec, tol):
+    """H2 / cc-pVDZ / cc-pVDZ-JKFIT, Cartesian: 10 basis functions (less than
+    one 16-column segment and one 64-row tile, every tile mostly padding), 46
+    auxiliary functions and one occupied orbital, in every precision mode."""
+    sym, xyz = ["H", "H"], [[0, 0, 0], [0, 0, 0.74]]
+    om = O.OracleMolecule.from_angstrom(sym, xyz, os.path.join(BASIS, "cc-pvdz.gbs"),
+                                        os.path.join(BASIS, "cc-pvdz-jkfit.gbs"))
+    r = O.rhf(om)
+    pm = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
+    assert (pm.nbf, pm.naux, om.nocc) == (10, 46, 1)
+    C = r["C"].T[:1]
+    J0, K0 = r["jk"](r["D"], C, 1)
+    with JKContext(pm.nbf, pm.naux) as ctx:
+        ctx.set_B(pm.eri3_packed(), pm.metric())
+        J, K = ctx.build(r["D"], C, 1, prec=prec)
+        J2, K2 = ctx.build(r["D"], C, 1, prec=prec)
+    dj, dk = np.abs(J - J0).max(), np.abs(K - K0).max()
+    say("H2", prec, "max|dJ|", dj, "max|dK|", dk)
+    assert dj < tol * max(1.0, np.abs(J0).max()) and dk < tol * max(1.0, np.abs(K0).max())
+    assert np.array_equal(J, J2) and np.array_equal(K, K2)
