@@ -578,14 +578,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"int8": "int8 digits, exact int32 acc (fp64 combine)", "fp32": "f32 (fp64 acc)",
-                  "bf16x6": "bf16x6 (fp32 split, fp64 acc)",
-                  "bf16x3": "bf16x3 (fp64 acc)", "fp64": "f64"}[prec],
+        "dtype": {"int8": "int8", "fp32": "f32", "bf16x6": "bf16", "bf16x3": "bf16", "fp64": "f64"}[prec],
         "data": "C20H42 geometry built in-repo; converged FP64 RI-RHF density and orbitals",
         "config": {
             "workload": f"RI-J/K build, all-trans C{args.alkane}H{2 * args.alkane + 2} cc-pVDZ/cc-pVDZ-JKFIT, "
                         f"nbf {nbf}, naux {naux}, nocc {nocc}, B FP64 resident in HBM",
             "precision": prec,
+            "arithmetic": {"int8": "int8 digit products, exact int32 class sums, fp64 combine; J from 35-bit "
+                                   "integers in fp64",
+                           "fp32": "f32 products, f32 chains of 16, fp64 sums; J in fp64",
+                           "bf16x6": "three-piece bf16 split of f32, six products, fp64 sums; J in fp64",
+                           "bf16x3": "two-piece bf16 split, three products, fp64 sums; J in fp64",
+                           "fp64": "f64"}[prec],
             "dE_vs_fp64_Eh": acc[prec]["dE"],
             "max_abs_dJ": acc[prec]["max_abs_dJ"],
             "max_abs_dK": acc[prec]["max_abs_dK"],
