@@ -447,8 +447,8 @@ def main():
                     help="instead of the headline, run the large-system RI-RHF on (H2O)_NWATER (seed 1) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU (aux-index sharded) design is "
-              "described in DESIGN.md and is not launched.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (--gpus 1); sharding the build over the auxiliary index "
+              "across GPUs is designed but not implemented, so no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:
