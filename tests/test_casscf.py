@@ -1,0 +1,112 @@
+"""The state-averaged CASSCF driver's host parts (nopt_amd/casscf.py), CPU only:
+the active-space CI against the closed form of two electrons in two orbitals,
+its reduced density matrices against its own energies, and the whole driver
+on C2H4 with a numpy RI-J/K from the same fitted B, against the energy NOPT
+prints for this input (examples/c2h4_2_2/benchmark.log:507, 542-543)."""
+import sys
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from nopt_amd.casscf import ActiveSpace, CASSCF
+from nopt_amd.molecule import Molecule
+from nopt_amd.scf import rhf
+from systems import SYSTEMS, CAS_C2H4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def random_integrals(n, naux, seed):
+    rng = np.random.default_rng(seed)
+    h = rng.standard_normal((n, n))
+    h = 0.5 * (h + h.T)
+    B = rng.standard_normal((naux, n, n))
+    B = 0.5 * (B + B.transpose(0, 2, 1))
+    return h, np.einsum("Qpq,Qrs->pqrs", B, B)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_two_electrons_in_two_orbitals_closed_form(seed):
+    """Singlets of 2 electrons in 2 orbitals from the three configuration
+    state functions (1^2, 2^2, the open-shell singlet) and the triplet
+    h11 + h22 + (11|22) - (12|12), written out by hand."""
+    h, g = random_integrals(2, 3, seed)
+    sp = ActiveSpace(2, 1, 1)
+    s2 = np.sqrt(2.0)
+    Hc = np.array([
+        [2 * h[0, 0] + g[0, 0, 0, 0], g[0, 1, 0, 1], s2 * (h[0, 1] + g[0, 0, 0, 1])],
+        [g[0, 1, 0, 1], 2 * h[1, 1] + g[1, 1, 1, 1], s2 * (h[0, 1] + g[1, 1, 1, 0])],
+        [s2 * (h[0, 1] + g[0, 0, 0, 1]), s2 * (h[0, 1] + g[1, 1, 1, 0]),
+         h[0, 0] + h[1, 1] + g[0, 0, 1, 1] + g[0, 1, 1, 0]]])
+    singlets = np.linalg.eigvalsh(Hc)
+    e_s, _, _, _ = sp.solve(h, g, 3, 1)
+    e_t, _, _, _ = sp.solve(h, g, 1, 3)
+    assert np.allclose(e_s, singlets, rtol=0, atol=1e-12)
+    assert abs(e_t[0] - (h[0, 0] + h[1, 1] + g[0, 0, 1, 1] - g[0, 1, 1, 0])) < 1e-12
+
+
+@pytest.mark.parametrize("n,na,nb,mult", [(3, 2, 1, 2), (4, 2, 2, 1), (4, 2, 2, 3), (5, 3, 2, 2)])
+def test_rdms_reproduce_ci_energies(n, na, nb, mult):
+    """Each root: E = sum h gamma + 1/2 sum (pq|rs) Gamma; trace gamma = N,
+    sum_pr Gamma_pprr = N (N - 1); <S^2> of the requested multiplicity."""
+    h, g = random_integrals(n, 4, n + na)
+    sp = ActiveSpace(n, na, nb)
+    e, V, _, _ = sp.solve(h, g, 2, mult)
+    N = na + nb
+    S = 0.5 * (mult - 1)
+    for k in range(2):
+        gam, Gam = sp.rdms(V[:, k])
+        assert abs(np.sum(h * gam) + 0.5 * np.sum(g * Gam) - e[k]) < 1e-11
+        assert abs(np.trace(gam) - N) < 1e-12
+        assert abs(np.einsum("pprr->", Gam) - N * (N - 1)) < 1e-11
+        assert abs(V[:, k] @ sp._s2 @ V[:, k] - S * (S + 1)) < 1e-10
+
+
+class _NumpyJK:
+    """RI-J/K over the host fit B = L^-1 (P|mn) (numpy, for the host test):
+    J(D) = sum_Q B_Q (B_Q . D), K = sum_Q (B_Q C^T)(B_Q C^T)^T."""
+
+    def __init__(self, m):
+        n = m.nbf
+        self.Bp = sl.solve_triangular(np.linalg.cholesky(m.metric()), m.eri3_packed(), lower=True)
+        r, c = np.tril_indices(n)
+        self.B = np.zeros((m.naux, n, n))
+        self.B[:, r, c] = self.Bp
+        self.B[:, c, r] = self.Bp
+
+    def build(self, D, C, nocc, prec="fp64"):
+        J = np.einsum("Qmn,Q->mn", self.B, np.einsum("Qrs,rs->Q", self.B, D)) if D is not None else None
+        X = self.B @ np.asarray(C)[:nocc].T
+        return J, np.einsum("Qmi,Qni->mn", X, X)
+
+    def build_batch(self, items, prec="fp64"):
+        return [self.build(*it) for it in items]
+
+    def fock_energy(self, H, D, J, K, cj=2.0, ck=1.0):
+        F = H + cj * J - ck * K
+        return F, float(np.sum(D * (F + H)))
+
+
+def test_sa_casscf_c2h4_with_host_jk():
+    """SA-CASSCF(2,2), two singlets with equal weights, from the RHF orbitals:
+    the averaged energy NOPT prints, and at the RHF orbitals its first CI."""
+    sym, xyz = SYSTEMS["C2H4"]
+    m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-ri")
+    ctx = _NumpyJK(m)
+    res = rhf(m, ctx=ctx, e_conv=1e-12, d_conv=1e-10)
+    cas = CASSCF(m, ctx.build_batch, ctx.Bp, n_core=m.nocc - 1, n_act=2, n_alpha=1, n_beta=1, nstates=2, mult=1)
+    r0 = cas.evaluate(res.C)
+    out = cas.run(res.C)
+    say("CI at RHF orbitals", r0["e_states"], "SA-CASSCF", out["E"], "states", out["e_states"], "iters",
+        out["iters"])
+    assert out["converged"]
+    # CI roots depend linearly on the orbitals, which NOPT's RHF fixes to its
+    # dE = 1e-9 stop (cas.inp:18): 1.2e-8 and 5e-10 here; the averaged
+    # CASSCF energy is stationary and is held to the printed digits
+    assert np.abs(r0["e_states"] - CAS_C2H4["ci_rhf"]).max() < 5e-8
+    assert abs(out["E"] - CAS_C2H4["E_avg"]) < 1e-9
+    assert np.abs(out["e_states"] - CAS_C2H4["states"]).max() < 1e-6

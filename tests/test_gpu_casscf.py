@@ -1,0 +1,73 @@
+"""State-averaged CASSCF(2,2) of C2H4 through the GPU build
+(nopt_amd/casscf.py): every J/K of the driver from the GPU and the active
+integrals from the B the GPU formed, against the energies NOPT prints for
+examples/c2h4_2_2/cas.inp (benchmark.log:507, 542-543); and NOPT's pair of
+one CASSCF step, (D_c, C_core) and (2 D_c + D_a, [C_core; NO sqrt(n/2)]) in
+one batched call (src/CAS.cpp:696-707, src/molecule.cpp:2925-2982,
+4407-4438), against the two builds the driver makes one at a time."""
+import sys
+
+import numpy as np
+import pytest
+
+from nopt_amd.casscf import CASSCF, gpu_jk
+from nopt_amd.jk import JKContext
+from nopt_amd.molecule import Molecule
+from nopt_amd.scf import rhf
+from systems import SYSTEMS, CAS_C2H4
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_c2h4(prec_stages):
+    """RHF, then SA-CASSCF(2,2) in stages: (prec, g_conv, e_conv) each, every
+    stage starting from the orbitals of the one before."""
+    sym, xyz = SYSTEMS["C2H4"]
+    m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-ri")
+    outs = []
+    with JKContext(m.nbf, m.naux) as ctx:
+        ctx.set_B(m.eri3_packed(), m.metric())
+        C = rhf(m, ctx=ctx, e_conv=1e-12, d_conv=1e-10).C
+        B = ctx.get_B()
+        for prec, g_conv, e_conv in prec_stages:
+            cas = CASSCF(m, gpu_jk(ctx, prec), B, n_core=m.nocc - 1, n_act=2, n_alpha=1, n_beta=1, nstates=2,
+                         mult=1)
+            out = cas.run(C, g_conv=g_conv, e_conv=e_conv)
+            out["pair"] = cas.pair_fock(out["C"], out["gamma"])
+            say(prec, "SA-CASSCF", out["E"], "diff", out["E"] - CAS_C2H4["E_avg"], "states", out["e_states"],
+                "iters", out["iters"], "gmax", out["gmax"])
+            outs.append(out)
+            C = out["C"]
+    return outs
+
+
+def pair_matches_single_builds(out):
+    FI, Ft = out["pair"]
+    dI = np.abs(FI - out["FI"]).max() / np.abs(out["FI"]).max()
+    dt = np.abs(Ft - out["Ft"]).max() / np.abs(out["Ft"]).max()
+    say("pair vs single builds", dI, dt)
+    return dI < 1e-12 and dt < 1e-12
+
+
+def test_sa_casscf_reproduces_nopt():
+    out, = run_c2h4([("fp64", 1e-7, 1e-11)])
+    assert out["converged"]
+    assert abs(out["E"] - CAS_C2H4["E_avg"]) < 1e-9
+    assert np.abs(out["e_states"] - CAS_C2H4["states"]).max() < 1e-6
+    assert pair_matches_single_builds(out)
+
+
+def test_sa_casscf_integer_builds_refined_in_fp64():
+    """Integer builds carry K to ~3e-8 relative, a deterministic rounding that
+    does not vary smoothly with the orbitals, so they take the orbitals to a
+    gradient of 1e-4 (within 1 micro-Hartree of the end), and FP64 builds
+    finish (the refinement the RHF driver does)."""
+    i8, f64 = run_c2h4([("int8", 1e-4, 1e-7), ("fp64", 1e-7, 1e-11)])
+    assert i8["converged"] and abs(i8["E"] - CAS_C2H4["E_avg"]) < 1e-6
+    assert pair_matches_single_builds(i8)
+    assert f64["converged"] and abs(f64["E"] - CAS_C2H4["E_avg"]) < 1e-9
