@@ -59,16 +59,17 @@ def _excite(det, i, j):
 
 class ActiveSpace:
     """Determinants of n_alpha + n_beta electrons in n orbitals and the
-    spin-summed excitation operators E_pq on CI vectors over them."""
+    spin-summed excitation operators E_pq (sparse) on CI vectors over them."""
 
     def __init__(self, n, n_alpha, n_beta):
         import itertools
-        self.n = n
+        import scipy.sparse as sp
+        self.n, self.nel = n, n_alpha + n_beta
         strings = lambda k: [sum(1 << p for p in c) for c in itertools.combinations(range(n), k)]
         self.dets = [a | (b << n) for a in strings(n_alpha) for b in strings(n_beta)]
         self.index = {d: i for i, d in enumerate(self.dets)}
         self.ms = 0.5 * (n_alpha - n_beta)
-        # E_pq as (rows, cols, signs) over determinant indices
+        nd = len(self.dets)
         self.E = {}
         for p in range(n):
             for q in range(n):
@@ -80,30 +81,23 @@ class ActiveSpace:
                             rows.append(self.index[r[1]])
                             cols.append(k)
                             sg.append(r[0])
-                self.E[p, q] = (np.array(rows, dtype=int), np.array(cols, dtype=int), np.array(sg))
-        # S_+ = sum_p a+_{p alpha} a_{p beta} maps out of the M_s space; S^2 on
-        # the space as S_- S_+ + S_z^2 + S_z, with S_- S_+ computed through the
-        # determinants of M_s + 1 on the fly
-        self._s2 = self._spin_squared()
+                self.E[p, q] = sp.csr_matrix((sg, (rows, cols)), shape=(nd, nd))
 
     def apply(self, p, q, c):
-        rows, cols, sg = self.E[p, q]
-        out = np.zeros_like(c)
-        np.add.at(out, rows, sg * c[cols])
-        return out
+        return self.E[p, q] @ c
 
-    def _spin_squared(self):
+    def spin_squared_matrix(self):
+        """<I|S^2|J> as S_- S_+ + S_z^2 + S_z over the determinants (dense;
+        small spaces)."""
         n, nd = self.n, len(self.dets)
         S2 = np.zeros((nd, nd))
         for k, d in enumerate(self.dets):
-            # S_+ |d>: a+_{p alpha} a_{p beta}
             up = {}
-            for p in range(n):
+            for p in range(n):  # S_+ = sum_p a+_{p alpha} a_{p beta}
                 r = _excite(d, p, p + n)
                 if r is not None:
                     up[r[1]] = up.get(r[1], 0.0) + r[0]
-            # S_- back: a+_{q beta} a_{q alpha}
-            for du, cu in up.items():
+            for du, cu in up.items():  # S_- = sum_q a+_{q beta} a_{q alpha}
                 for q in range(n):
                     r = _excite(du, q + n, q)
                     if r is not None:
@@ -111,57 +105,73 @@ class ActiveSpace:
             S2[k, k] += self.ms * self.ms + self.ms
         return S2
 
-    def hamiltonian(self, h, g):
-        """<I| sum h_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - d_qr E_ps) |J>; the
-        d_qr term folds into the one-electron part as -1/2 sum_q (pq|qs)."""
-        n, nd = self.n, len(self.dets)
+    def spin_squared(self, c):
+        """<c|S^2|c> = N - N^2/4 - 1/2 sum_pq Gamma_pqqp for a normalised c,
+        with sum_pq Gamma_pqqp = sum_pq |E_pq c|^2 - n N."""
+        N, n = self.nel, self.n
+        t = sum(float(np.dot(v, v)) for v in (self.E[p, q] @ c for p in range(n) for q in range(n)))
+        return N - 0.25 * N * N - 0.5 * (t - n * N)
+
+    def sigma(self, h, g, c):
+        """(sum h_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - d_qr E_ps)) c for a
+        vector or the columns of a matrix; the d_qr term folds into the
+        one-electron part as -1/2 sum_q (pq|qs)."""
+        n = self.n
         hk = h - 0.5 * np.einsum("pqqs->ps", g)
-        Hm = np.zeros((nd, nd))
-        for k in range(nd):
-            e = np.zeros(nd)
-            e[k] = 1.0
-            Ers = {(r, s): self.apply(r, s, e) for r in range(n) for s in range(n)}
-            col = np.zeros(nd)
-            for p in range(n):
-                for q in range(n):
-                    col += hk[p, q] * Ers[p, q]
-                    for r in range(n):
-                        for s in range(n):
-                            if g[p, q, r, s] != 0.0:
-                                col += 0.5 * g[p, q, r, s] * self.apply(p, q, Ers[r, s])
-            Hm[:, k] = col
+        V = np.stack([self.E[r, s] @ c for r in range(n) for s in range(n)])  # row r*n+s: E_rs c
+        out = np.tensordot(hk.reshape(n * n), V, axes=(0, 0))
+        W = np.tensordot(0.5 * g.reshape(n * n, n * n), V, axes=(1, 0))
+        for p in range(n):
+            for q in range(n):
+                out += self.E[p, q] @ W[p * n + q]
+        return out
+
+    def hamiltonian(self, h, g):
+        """The CI matrix, dense (small spaces)."""
+        Hm = self.sigma(h, g, np.eye(len(self.dets)))
         return 0.5 * (Hm + Hm.T)
 
     def rdms(self, c):
-        """gamma_pq = <c|E_pq|c>, Gamma_pqrs = <c|E_pq E_rs|c> - d_qr gamma_ps."""
+        """gamma_pq = <c|E_pq|c>, Gamma_pqrs = <c|E_pq E_rs|c> - d_qr gamma_ps
+        (<c|E_pq E_rs|c> = <E_qp c|E_rs c>)."""
         n = self.n
-        Ec = {(r, s): self.apply(r, s, c) for r in range(n) for s in range(n)}
-        gam = np.array([[c @ Ec[p, q] for q in range(n)] for p in range(n)])
-        Gam = np.zeros((n, n, n, n))
-        for p in range(n):
-            for q in range(n):
-                # <c|E_pq E_rs|c> = <E_qp c|E_rs c>
-                for r in range(n):
-                    for s in range(n):
-                        Gam[p, q, r, s] = Ec[q, p] @ Ec[r, s]
-                Gam[p, q, q, :] -= gam[p, :]
+        V = np.stack([self.E[r, s] @ c for r in range(n) for s in range(n)])  # row r*n+s
+        gam = (V @ c).reshape(n, n)
+        VT = V.reshape(n, n, -1).transpose(1, 0, 2).reshape(n * n, -1)    # row p*n+q = E_qp c
+        Gam = (VT @ V.T).reshape(n, n, n, n)
+        for q in range(n):
+            Gam[:, q, q, :] -= gam
         return gam, Gam
 
-    def solve(self, h, g, nstates, mult, weights=None):
-        """Lowest `nstates` eigenpairs of spin (mult - 1)/2; returns energies
-        (without the core energy), vectors and the weighted RDMs."""
-        Hm = self.hamiltonian(h, g)
-        w, V = np.linalg.eigh(Hm)
+    def solve(self, h, g, nstates, mult, weights=None, guess=None):
+        """Lowest `nstates` roots of spin (mult - 1)/2: energies (without the
+        core energy), vectors (columns) and the weighted RDMs. Dense
+        diagonalisation up to 400 determinants, Lanczos (eigsh) beyond,
+        asking for more roots until enough have the requested spin."""
+        nd = len(self.dets)
         S = 0.5 * (mult - 1)
-        s2 = np.array([V[:, k] @ self._s2 @ V[:, k] for k in range(len(w))])
-        pick = [k for k in range(len(w)) if abs(s2[k] - S * (S + 1)) < 1e-6][:nstates]
+        if nd <= 400:
+            w, V = np.linalg.eigh(self.hamiltonian(h, g))
+        else:
+            import scipy.sparse.linalg as sla
+            op = sla.LinearOperator((nd, nd), matvec=lambda x: self.sigma(h, g, x), dtype=np.float64)
+            k = nstates + 4
+            while True:
+                v0 = guess[:, 0] if guess is not None else None
+                w, V = sla.eigsh(op, k=k, which="SA", v0=v0, tol=1e-13)
+                o = np.argsort(w)
+                w, V = w[o], V[:, o]
+                if sum(abs(self.spin_squared(V[:, j]) - S * (S + 1)) < 1e-6 for j in range(k)) >= nstates:
+                    break
+                k = min(2 * k, nd - 1)
+        pick = [j for j in range(len(w)) if abs(self.spin_squared(V[:, j]) - S * (S + 1)) < 1e-6][:nstates]
         if len(pick) < nstates:
             raise ValueError("fewer states of the requested spin than nstates")
         wts = np.ones(nstates) / nstates if weights is None else np.asarray(weights, float) / np.sum(weights)
         gam = np.zeros((self.n, self.n))
         Gam = np.zeros((self.n,) * 4)
-        for wk, k in zip(wts, pick):
-            g1, g2 = self.rdms(V[:, k])
+        for wk, j in zip(wts, pick):
+            g1, g2 = self.rdms(V[:, j])
             gam += wk * g1
             Gam += wk * g2
         return w[pick], V[:, pick], gam, Gam
@@ -197,6 +207,7 @@ class CASSCF:
             _, T, V = mol.one_electron()
             H = T + V
         self.H = H
+        self._guess = None
 
     def natural_factor(self, gam, Ca):
         """Rows NO_k sqrt(n_k / 2) of gamma's natural orbitals (nat_orb_calc
@@ -231,7 +242,9 @@ class CASSCF:
         g_aaaa = np.einsum("Qtu,Qvw->tuvw", Baa, Baa, optimize=True)
         FI_mo = C.T @ FI @ C
         h_act = FI_mo[nc:nc + na, nc:nc + na]
-        eci, vecs, gam, Gam = self.space.solve(h_act, g_aaaa, self.nstates, self.mult, self.weights)
+        eci, vecs, gam, Gam = self.space.solve(h_act, g_aaaa, self.nstates, self.mult, self.weights,
+                                               guess=self._guess)
+        self._guess = vecs  # Lanczos starts from the last ground state
         E = e_core + float(np.sum(h_act * gam)) + 0.5 * float(np.sum(g_aaaa * Gam))
         # total Fock from the second build of the pair
         Da = Ca @ gam @ Ca.T

@@ -71,4 +71,12 @@ GOLDEN_E = {
 CAS_C2H4 = {"ci_rhf": [-78.0595141689, -77.6877369591], "E_avg": -77.8755259203,
             "states": [-78.0575925136, -77.6934593271]}
 
+# NOPT's state-averaged CASSCF(8,8) of octatetraene (examples/dmrg/4ene/
+# 4ene_CAS.inp: three singlets, weights 1 1 1, from 4ene_RHF.orb): the CI
+# roots at the file's orbitals (4ene_CAS.log:1326-1328), the averaged energy
+# (4ene_CAS.log:1349, orbital gradient 7.8e-7) and the states at convergence
+# (4ene_CAS.log:1408-1410)
+CAS_4ENE = {"ci_file": [-308.7964933374, -308.6103724502, -308.5702725368], "E_avg": -308.6944759121,
+            "states": [-308.8261097128, -308.6493233879, -308.6079946356]}
+
 SYSTEMS = {"C2H4": C2H4, "C6H8": C6H8, "guanine": GUANINE, "pBQ": PBQ}

@@ -56,6 +56,7 @@ def test_rdms_reproduce_ci_energies(n, na, nb, mult):
     h, g = random_integrals(n, 4, n + na)
     sp = ActiveSpace(n, na, nb)
     e, V, _, _ = sp.solve(h, g, 2, mult)
+    S2 = sp.spin_squared_matrix()
     N = na + nb
     S = 0.5 * (mult - 1)
     for k in range(2):
@@ -63,7 +64,24 @@ def test_rdms_reproduce_ci_energies(n, na, nb, mult):
         assert abs(np.sum(h * gam) + 0.5 * np.sum(g * Gam) - e[k]) < 1e-11
         assert abs(np.trace(gam) - N) < 1e-12
         assert abs(np.einsum("pprr->", Gam) - N * (N - 1)) < 1e-11
-        assert abs(V[:, k] @ sp._s2 @ V[:, k] - S * (S + 1)) < 1e-10
+        assert abs(V[:, k] @ S2 @ V[:, k] - S * (S + 1)) < 1e-10
+        assert abs(sp.spin_squared(V[:, k]) - S * (S + 1)) < 1e-10
+
+
+def test_lanczos_matches_dense():
+    """Beyond 400 determinants the CI is solved by Lanczos on the sigma
+    product: 6 electrons in 7 orbitals (1225 determinants), the three lowest
+    singlets and doublet-free triplets against a dense diagonalisation of
+    the same matrix."""
+    h, g = random_integrals(7, 6, 11)
+    sp = ActiveSpace(7, 3, 3)
+    w, V = np.linalg.eigh(sp.hamiltonian(h, g))
+    s2 = np.array([sp.spin_squared(V[:, k]) for k in range(12)])
+    for mult, S in ((1, 0.0), (3, 1.0)):
+        ref = w[:12][np.abs(s2 - S * (S + 1)) < 1e-6][:3]
+        e, _, _, _ = sp.solve(h, g, 3, mult)
+        say("mult", mult, "Lanczos", e, "dense", ref)
+        assert np.allclose(e, ref, rtol=0, atol=1e-10)
 
 
 class _NumpyJK:
