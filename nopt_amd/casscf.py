@@ -33,8 +33,12 @@ JKContext.build_batch: (D, C rows, nocc). `gpu_jk(ctx, prec)` gives the GPU
 build; the pair (D_c, C_core) and (2 D_c + D_a, [C_core; NO sqrt(n/2)]) is one
 batched call when gamma is known (`pair_fock`).
 """
+import itertools
+
 import numpy as np
 import scipy.linalg as sl
+import scipy.sparse as sparse
+import scipy.sparse.linalg as sla
 
 from .molecule import unpack_sym
 
@@ -62,8 +66,6 @@ class ActiveSpace:
     spin-summed excitation operators E_pq (sparse) on CI vectors over them."""
 
     def __init__(self, n, n_alpha, n_beta):
-        import itertools
-        import scipy.sparse as sp
         self.n, self.nel = n, n_alpha + n_beta
         strings = lambda k: [sum(1 << p for p in c) for c in itertools.combinations(range(n), k)]
         self.dets = [a | (b << n) for a in strings(n_alpha) for b in strings(n_beta)]
@@ -81,7 +83,7 @@ class ActiveSpace:
                             rows.append(self.index[r[1]])
                             cols.append(k)
                             sg.append(r[0])
-                self.E[p, q] = sp.csr_matrix((sg, (rows, cols)), shape=(nd, nd))
+                self.E[p, q] = sparse.csr_matrix((sg, (rows, cols)), shape=(nd, nd))
 
     def apply(self, p, q, c):
         return self.E[p, q] @ c
@@ -153,7 +155,6 @@ class ActiveSpace:
         if nd <= 400:
             w, V = np.linalg.eigh(self.hamiltonian(h, g))
         else:
-            import scipy.sparse.linalg as sla
             op = sla.LinearOperator((nd, nd), matvec=lambda x: self.sigma(h, g, x), dtype=np.float64)
             k = nstates + 4
             while True:
