@@ -79,4 +79,9 @@ CAS_C2H4 = {"ci_rhf": [-78.0595141689, -77.6877369591], "E_avg": -77.8755259203,
 CAS_4ENE = {"ci_file": [-308.7964933374, -308.6103724502, -308.5702725368], "E_avg": -308.6944759121,
             "states": [-308.8261097128, -308.6493233879, -308.6079946356]}
 
+# NOPT's CASSCF of the p-benzoquinone anion (examples/dmrg/pBQ/pBQ_2A_CAS.inp:
+# charge -1, doublets, two states, from pBQ_2B2_RHF.orb): the CI roots at the
+# file's orbitals (pBQ_2A_CAS.log:1065-1066)
+CAS_PBQ = {"ci_file": [-379.2978536692, -379.2148560831]}
+
 SYSTEMS = {"C2H4": C2H4, "C6H8": C6H8, "guanine": GUANINE, "pBQ": PBQ}

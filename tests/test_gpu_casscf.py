@@ -14,7 +14,7 @@ from nopt_amd.casscf import CASSCF, gpu_jk
 from nopt_amd.jk import JKContext
 from nopt_amd.molecule import Molecule
 from nopt_amd.scf import rhf
-from systems import SYSTEMS, CAS_C2H4, CAS_4ENE, ENE4, ENE4_ORB
+from systems import SYSTEMS, CAS_C2H4, CAS_4ENE, CAS_PBQ, ENE4, ENE4_ORB, PBQ, PBQ_ORB
 
 pytestmark = pytest.mark.gpu
 
@@ -100,3 +100,26 @@ def test_sa_casscf_octatetraene_from_nopt_orbitals():
     assert np.abs(r0["e_states"] - CAS_4ENE["ci_file"]).max() < 5e-8
     assert out["converged"] and abs(out["E"] - CAS_4ENE["E_avg"]) < 5e-8
     assert np.abs(out["e_states"] - CAS_4ENE["states"]).max() < 1e-6
+
+
+def test_doublet_anion_ci_at_nopt_orbitals():
+    """NOPT's CASSCF of the p-benzoquinone anion (examples/dmrg/pBQ/
+    pBQ_2A_CAS.inp: charge -1, 13 electrons in 10 active orbitals, doublets,
+    two states, cc-pVDZ / cc-pVDZ-JKFIT, 25200 determinants) at its starting
+    orbitals, NOPT's pBQ_2B2_RHF.orb with active orbitals 21, 24-31, 36
+    (1-based): both CI roots through the GPU build against the log. They sit
+    9.0e-9 Eh below NOPT's, an offset of the size seen for octatetraene."""
+    from nopt_amd.orbfile import read_orb, gamess_to_libint
+    m = Molecule.from_angstrom(*PBQ, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
+    C = gamess_to_libint(read_orb(PBQ_ORB).C, m.function_l()).T
+    act = [20, 23, 24, 25, 26, 27, 28, 29, 30, 35]
+    core = [i for i in range(28) if i not in act]  # (56 + 1 - 13) / 2 = 22 core orbitals
+    assert len(core) == 22
+    C = C[:, core + act + [i for i in range(C.shape[1]) if i not in act and i not in core]]
+    with JKContext(m.nbf, m.naux) as ctx:
+        ctx.set_B(m.eri3_packed(), m.metric())
+        cas = CASSCF(m, gpu_jk(ctx, "fp64"), ctx.get_B(), n_core=22, n_act=10, n_alpha=7, n_beta=6, nstates=2,
+                     mult=2)
+        r = cas.evaluate(C)
+    say("pBQ anion roots", r["e_states"], "minus log", r["e_states"] - CAS_PBQ["ci_file"])
+    assert np.abs(r["e_states"] - CAS_PBQ["ci_file"]).max() < 2e-8
