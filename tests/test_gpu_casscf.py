@@ -78,10 +78,12 @@ def test_sa_casscf_octatetraene_from_nopt_orbitals():
     three singlets, equal weights, cc-pVDZ / cc-pVDZ-JKFIT, 170 Cartesian AOs,
     4900 determinants) from NOPT's RHF orbitals (its .orb file) with active
     orbitals 26-31, 37 and 39 (1-based, 4ene_CAS.inp:29): the CI roots at those
-    orbitals and the converged averaged energy against the log. At NOPT's own
-    orbitals the roots sit 0.8-3.1e-8 Eh below NOPT's printed ones (the V^-1/2
-    versus Cholesky form of the fit accounts for 1.7e-10 of that, the basis
-    data are NOPT's), so this case is held to 5e-8."""
+    orbitals and the converged averaged energy against the log. NOPT's
+    Davidson solver stops at a residual norm of 5e-8 (4ene_CAS.log:1093), so
+    its printed roots lie above the exact ones by up to that much: at NOPT's
+    orbitals the exact roots are lower by 0.8e-8, 3.0e-8 and 0.7e-8 Eh,
+    following NOPT's final residual norms 1.1e-8, 3.0e-8 and 1.3e-8. This
+    case is held to 5e-8."""
     from nopt_amd.orbfile import read_orb, gamess_to_libint
     m = Molecule.from_angstrom(*ENE4, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
     C = gamess_to_libint(read_orb(ENE4_ORB).C, m.function_l()).T
@@ -107,8 +109,10 @@ def test_doublet_anion_ci_at_nopt_orbitals():
     pBQ_2A_CAS.inp: charge -1, 13 electrons in 10 active orbitals, doublets,
     two states, cc-pVDZ / cc-pVDZ-JKFIT, 25200 determinants) at its starting
     orbitals, NOPT's pBQ_2B2_RHF.orb with active orbitals 21, 24-31, 36
-    (1-based): both CI roots through the GPU build against the log. They sit
-    9.0e-9 Eh below NOPT's, an offset of the size seen for octatetraene."""
+    (1-based): both CI roots through the GPU build against the log. The exact
+    roots are 9.0e-9 Eh below NOPT's, whose Davidson solver stopped at
+    residual norms of 1.5e-8 and 1.7e-8 (a root stopped early lies above the
+    exact one)."""
     from nopt_amd.orbfile import read_orb, gamess_to_libint
     m = Molecule.from_angstrom(*PBQ, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
     C = gamess_to_libint(read_orb(PBQ_ORB).C, m.function_l()).T
