@@ -86,6 +86,15 @@ int nopt_jk_get_B(nopt_jk_ctx* ctx, double* B_packed);
 /* Device bytes held by the context. */
 long nopt_jk_device_bytes(const nopt_jk_ctx* ctx);
 
+/* Active-space integrals through the fitted B, FP64 (the MO_INTS half of
+ * calc_2el_MO_INTS_RI, src/RI.cpp:197-226):
+ *   ints[(a2*na + a1)*na*na + b2*na + b1] = sum_Q R_Q[a2][a1] R_Q[b2][b1],
+ *   R_Q[a2][a1] = sum_mn C1[a1][m] B_Q,mn C2[a2][n],
+ * C1 and C2 na x nbf with orbitals as rows, 1 <= na <= 32. Needs the FP64
+ * form of B (NOPT_ESTATE for a context made with NOPT_CTX_INT8_ONLY). Host
+ * pointers; synchronous. */
+int nopt_jk_active_ints(nopt_jk_ctx* ctx, const double* C1, const double* C2, long na, double* ints);
+
 /* One RI-J/K build, host buffers: J_mn = sum_Q B_Qmn sum_rs B_Qrs D_rs and
  * K_mn = sum_Q sum_{i<nocc} (sum_r B_Qmr C_ir)(sum_s B_Qns C_is).
  * J and K (nbf x nbf) are overwritten. Either output may be NULL to skip it;

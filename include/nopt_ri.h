@@ -49,11 +49,12 @@ int regen_RI_AA_release();
 // NOPT's DM_to_F_transform_RI (src/RI.cpp:115-176).
 int DM_to_F_transform_RI(double* J, double* K, double* DM, double* MO, long n_c, long n_ao);
 
-// The inactive J/K half of NOPT's calc_2el_MO_INTS_RI (src/RI.cpp:229-290):
-// J from DM_D and K from the first n_c rows of MO_VEC; returns 0 without
-// touching J and K when either is NULL, like NOPT. The active-space
-// integrals (tu|vw) that NOPT also forms are not produced (MO_INTS, MO_VEC1,
-// MO_VEC2, n_a and t are accepted and ignored).
+// NOPT's calc_2el_MO_INTS_RI (src/RI.cpp:178-292): first the active-space
+// integrals MO_INTS[(a2 n_a + a1) n_a^2 + b2 n_a + b1] = sum_Q R_Q[a2][a1]
+// R_Q[b2][b1], R_Q[a2][a1] = sum_mn MO_VEC1[a1][m] B_Q,mn MO_VEC2[a2][n]
+// (orbitals as rows of n_ao, or as columns of n_a when t != 0; n_a <= 32;
+// skipped when MO_INTS is NULL), then, unless J or K is NULL (returns 0 there,
+// like NOPT), J from DM_D and K from the first n_c rows of MO_VEC.
 int calc_2el_MO_INTS_RI(long n_ao, double* DM_D, double* J, double* K, double* MO_INTS, double* MO_VEC,
                         double* MO_VEC1, double* MO_VEC2, long n_c, long n_a, long t);
 

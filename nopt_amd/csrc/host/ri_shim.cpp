@@ -109,13 +109,34 @@ int DM_to_F_transform_RI(double* J, double* K, double* DM, double* MO, long n_c,
 
 int calc_2el_MO_INTS_RI(long n_ao, double* DM_D, double* J, double* K, double* MO_INTS, double* MO_VEC,
                         double* MO_VEC1, double* MO_VEC2, long n_c, long n_a, long t) {
-  (void)MO_INTS;
-  (void)MO_VEC1;
-  (void)MO_VEC2;
-  (void)n_a;
-  (void)t;
-  if (J == nullptr || K == nullptr) return 0;  // src/RI.cpp:230-233 returns before J and K as well
   nopt::clear_error();
+  if (MO_INTS != nullptr) {  // src/RI.cpp:197-226, before the J/K early return
+    if (!g_ctx) return -nopt::set_error(NOPT_ESTATE, "calc_2el_MO_INTS_RI: no RI context");
+    if (n_ao != g_n_ao) return -nopt::set_error(NOPT_EINVAL, "calc_2el_MO_INTS_RI: n_ao does not match");
+    if (!MO_VEC1 || !MO_VEC2 || n_a < 1)
+      return -nopt::set_error(NOPT_EINVAL, "calc_2el_MO_INTS_RI: MO_INTS needs MO_VEC1, MO_VEC2 and n_a >= 1");
+    std::vector<double> c1, c2;
+    const double* r1 = MO_VEC1;
+    const double* r2 = MO_VEC2;
+    if (t) {  // orbitals as columns, [m][a] with leading dimension n_a: rows for the device
+      try {
+        c1.resize((size_t)n_a * n_ao);
+        c2.resize((size_t)n_a * n_ao);
+      } catch (const std::bad_alloc&) {
+        return -nopt::set_error(NOPT_ENOMEM, "calc_2el_MO_INTS_RI: host allocation failed");
+      }
+      for (long m = 0; m < n_ao; ++m)
+        for (long a = 0; a < n_a; ++a) {
+          c1[(size_t)a * n_ao + m] = MO_VEC1[m * n_a + a];
+          c2[(size_t)a * n_ao + m] = MO_VEC2[m * n_a + a];
+        }
+      r1 = c1.data();
+      r2 = c2.data();
+    }
+    const int rc = nopt_jk_active_ints(g_ctx, r1, r2, n_a, MO_INTS);
+    if (rc) return -rc;
+  }
+  if (J == nullptr || K == nullptr) return 0;  // src/RI.cpp:230-233 returns before J and K as well
   if (!g_ctx) return -nopt::set_error(NOPT_ESTATE, "calc_2el_MO_INTS_RI: no RI context");
   if (n_ao != g_n_ao) return -nopt::set_error(NOPT_EINVAL, "calc_2el_MO_INTS_RI: n_ao does not match");
   const int rc = nopt_jk_build(g_ctx, DM_D, MO_VEC, n_c, J, K, g_prec);

@@ -57,6 +57,70 @@ __global__ void k_pack_B(const double* __restrict__ B64, long ld64, int nbf, lon
   Bp[e] = B64[(Q * nbf + a) * ld64 + b];
 }
 
+// ----------------------------------------------------- active-space integrals
+// R[Q][a2*na + a1] = sum_m C1[a1][m] sum_n B_Q[m][n] C2[a2][n] (C1, C2: na x
+// nbf, orbitals as rows). One block per Q; each wave takes rows m = w, w+4, ...
+// with its lanes over n, sums the row's na products across the wave (butterfly,
+// then lane 0's value for all lanes, so every lane holds the same sum), and
+// each lane adds its share of the na x na outer product with column m of C1.
+// The four waves' partial R are added in a fixed order.
+template <int NA>
+__global__ __launch_bounds__(256) void k_active_half(const double* __restrict__ B64, long ld64, int nbf,
+                                                     const double* __restrict__ C1, const double* __restrict__ C2,
+                                                     int na, double* __restrict__ R) {
+  constexpr int PER = (NA * NA + 63) / 64;
+  __shared__ double part[4][PER * 64];
+  const int Q = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const double* Bq = B64 + (long)Q * nbf * ld64;
+  double r[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) r[k] = 0.0;
+  for (int m = w; m < nbf; m += 4) {
+    double u[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) u[a] = 0.0;
+    for (int n = lane; n < nbf; n += 64) {
+      const double b = Bq[(long)m * ld64 + n];
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+        if (a < na) u[a] += b * C2[(long)a * nbf + n];
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      double v = u[a];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      u[a] = __shfl(v, 0, 64);
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = lane + 64 * k, a2 = e / NA, a1 = e % NA;
+      if (a2 < na && a1 < na) {
+        double ua = 0.0;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) ua = (a == a2) ? u[a] : ua;  // register select, no dynamic index
+        r[k] += C1[(long)a1 * nbf + m] * ua;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < PER; ++k) part[w][lane + 64 * k] = r[k];
+  __syncthreads();
+  for (int e = threadIdx.x; e < na * na; e += 256) {
+    const int i = (e / na) * NA + e % na;
+    R[(long)Q * na * na + e] = ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
+  }
+}
+
+// ints[e1][e2] = sum_Q R[Q][e1] R[Q][e2], e = a2*na + a1 (nn = na*na).
+__global__ void k_active_ints(const double* __restrict__ R, long naux, int nn, double* __restrict__ ints) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nn * nn) return;
+  const int e1 = (int)(idx / nn), e2 = (int)(idx % nn);
+  double s = 0.0;
+  for (long Q = 0; Q < naux; ++Q) s += R[Q * nn + e1] * R[Q * nn + e2];
+  ints[idx] = s;
+}
+
 // dst[r][c] (ldd) = src[r][c] (lds) for r < rows, c < cols; zero elsewhere
 // in [0, rows_pad) x [0, ldd)
 template <typename T>
@@ -1535,6 +1599,49 @@ int nopt_jk_get_B(nopt_jk_ctx* c, double* Bp) {
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   dev_free(dA);
   if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("nopt_jk_get_B: ") + hip_msg(e));
+  return NOPT_OK;
+}
+
+int nopt_jk_active_ints(nopt_jk_ctx* c, const double* C1, const double* C2, long na, double* ints) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  if (!C1 || !C2 || !ints) return set_error(NOPT_EINVAL, "nopt_jk_active_ints: null pointer");
+  if (na < 1 || na > 32) return set_error(NOPT_EINVAL, "nopt_jk_active_ints: need 1 <= na <= 32");
+  if (!c->B64) return set_error(NOPT_ESTATE, "nopt_jk_active_ints: the context keeps no FP64 B (NOPT_CTX_INT8_ONLY)");
+  const long nbf = c->nbf, naux = c->naux, nn = na * na;
+  double *dC = nullptr, *dR = nullptr, *dI = nullptr;
+  rc = dev_alloc((void**)&dC, sizeof(double) * 2 * na * nbf, "active orbitals");
+  if (!rc) rc = dev_alloc((void**)&dR, sizeof(double) * naux * nn, "active half-transform");
+  if (!rc) rc = dev_alloc((void**)&dI, sizeof(double) * nn * nn, "active integrals");
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    e = hipMemcpyAsync(dC, C1, sizeof(double) * na * nbf, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dC + na * nbf, C2, sizeof(double) * na * nbf, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+      const double* d1 = dC;
+      const double* d2 = dC + na * nbf;
+      if (na <= 8)
+        k_active_half<8><<<(unsigned)naux, 256, 0, c->stream>>>(c->B64, c->ld64, (int)nbf, d1, d2, (int)na, dR);
+      else if (na <= 16)
+        k_active_half<16><<<(unsigned)naux, 256, 0, c->stream>>>(c->B64, c->ld64, (int)nbf, d1, d2, (int)na, dR);
+      else
+        k_active_half<32><<<(unsigned)naux, 256, 0, c->stream>>>(c->B64, c->ld64, (int)nbf, d1, d2, (int)na, dR);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      k_active_ints<<<(unsigned)((nn * nn + 255) / 256), 256, 0, c->stream>>>(dR, naux, (int)nn, dI);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(ints, dI, sizeof(double) * nn * nn, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  dev_free(dC);
+  dev_free(dR);
+  dev_free(dI);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("nopt_jk_active_ints: ") + hip_msg(e));
   return NOPT_OK;
 }
 

@@ -81,6 +81,21 @@ class JKContext:
         L.check(L.lib().nopt_jk_get_B(self._h, L.dptr(out)))
         return out
 
+    def active_ints(self, C1, C2=None):
+        """(t u|v w) over the fitted B for active orbitals as rows of C1 and C2
+        (na x nbf; C2 = C1 by default): ints[t2, t1, u2, u1] = sum_Q
+        R_Q[t2, t1] R_Q[u2, u1], R_Q[t2, t1] = C1[t1] B_Q C2[t2]^T (the MO_INTS
+        of NOPT's calc_2el_MO_INTS_RI)."""
+        n = self.nbf
+        C1 = np.ascontiguousarray(C1, dtype=np.float64)
+        C2 = C1 if C2 is None else np.ascontiguousarray(C2, dtype=np.float64)
+        if C1.ndim != 2 or C1.shape[1] != n or C2.shape != C1.shape:
+            raise ValueError("C1 and C2 must both be (na, nbf)")
+        na = C1.shape[0]
+        out = np.empty((na, na, na, na))
+        L.check(L.lib().nopt_jk_active_ints(self._h, L.dptr(C1), L.dptr(C2), na, L.dptr(out)))
+        return out
+
     def device_bytes(self):
         return int(L.lib().nopt_jk_device_bytes(self._h))
 

@@ -361,6 +361,20 @@ def rhf(mol, e_conv=1e-11, d_conv=1e-9, max_it=100, verbose=False):
     return dict(E=E, C=Cc, D=D, F=F, H=H, S=S, J=J, K=K, jk=jk, d5=Td, iters=it, dropped=dropped)
 
 
+def nopt_mo_ints(RI_AA, Vm05, VEC1, VEC2):
+    """The MO_INTS half of calc_2el_MO_INTS_RI (src/RI.cpp:197-226), t = 0:
+    RI_aA = VEC1 . RI_AA (:197-201), times V^-1/2 (:203-207), [a][n][Q] ->
+    [n][a][Q] (:209-215), RI_aa = VEC2 . RI_aA (:217-221), MO_INTS =
+    RI_aa RI_aa^T (:223-226), as an (na, na, na, na) array
+    [a2, a1, b2, b1]."""
+    n, naux = RI_AA.shape[0], RI_AA.shape[2]
+    na = VEC1.shape[0]
+    aA = (VEC1 @ RI_AA.reshape(n, n * naux)).reshape(na * n, naux) @ Vm05
+    aA = aA.reshape(na, n, naux).transpose(1, 0, 2).reshape(n, na * naux)
+    aa = (VEC2 @ aA).reshape(na * na, naux)
+    return (aa @ aa.T).reshape(na, na, na, na)
+
+
 def brute_force_jk(eri3, V, D):
     """Explicit four-index RI contraction, tiny systems only:
     (mn|rs)_RI = sum_PQ (mn|P) V^-1_PQ (Q|rs), formed as W^T W with

@@ -171,6 +171,39 @@ def test_nopt_ri_entry_points_match_oracle(c2h4_case):
                                     om.nocc, n) < 0
 
 
+@pytest.mark.parametrize("t", [0, 1])
+def test_nopt_mo_ints_through_the_shim(c2h4_case, t):
+    """The MO_INTS half of calc_2el_MO_INTS_RI through include/nopt_ri.h
+    (orbitals as rows, t = 0, or as columns, t = 1; two different random
+    active sets of 6) against the oracle's restatement of src/RI.cpp:197-226,
+    and nopt_jk_active_ints through JKContext against the same; J and K stay
+    untouched when they are NULL."""
+    from nopt_amd import _lib as L
+    om, r, pm = c2h4_case
+    lib = L.lib()
+    n, na = pm.nbf, 6
+    rng = np.random.default_rng(5)
+    V1, V2 = rng.standard_normal((na, n)), rng.standard_normal((na, n))
+    ref = O.nopt_mo_ints(om.eri3_nopt(), O.s05(om.metric())[0], V1, V2)
+    V, A = pm.metric(), np.ascontiguousarray(pm.eri3_nopt())
+    assert lib.RI_gpu_attach(L.dptr(V), L.dptr(A), n, pm.naux, 0) == 0
+    try:
+        out = np.full((na, na, na, na), 7.0)
+        a1 = np.ascontiguousarray(V1.T if t else V1)
+        a2 = np.ascontiguousarray(V2.T if t else V2)
+        assert lib.calc_2el_MO_INTS_RI(n, None, None, None, L.dptr(out), None, L.dptr(a1), L.dptr(a2), 0, na,
+                                       t) == 0
+    finally:
+        lib.regen_RI_AA_release()
+    with JKContext(n, pm.naux) as ctx:
+        ctx.set_B(pm.eri3_packed(), pm.metric())
+        out2 = ctx.active_ints(V1, V2)
+    scale = np.abs(ref).max()
+    say("MO_INTS t", t, "shim rel", np.abs(out - ref).max() / scale, "JKContext rel", np.abs(out2 - ref).max() / scale)
+    assert np.abs(out - ref).max() < 1e-10 * scale
+    assert np.abs(out2 - ref).max() < 1e-10 * scale
+
+
 # (system, orbital file, tolerance on the orbital energies): the file stores
 # energies to NOPT's convergence, and the FP64 build at the file's orbitals
 # differs from them by 5.3e-7 (pBQ) and 2.8e-6 (octatetraene, the oracle's

@@ -180,3 +180,25 @@ def test_prepared_b_variant_matches_nopt_chain_and_brute_force():
     Jb, Kb = O.brute_force_jk(eri3, m.metric(), r["D"])
     assert np.abs(J - J0).max() < 1e-11 and np.abs(K - K0).max() < 1e-11
     assert np.abs(J - Jb).max() < 1e-11 and np.abs(K - Kb).max() < 1e-11
+
+
+def test_nopt_mo_ints_against_four_index_ri():
+    """The oracle's MO_INTS (NOPT's V^-1/2 chain) against the explicit
+    four-index RI integrals through a Cholesky fit, (mn|rs) = W^T W with
+    W = L^-1 (P|mn), transformed with two different random orbital sets:
+    MO_INTS[a2, a1, b2, b1] = sum VEC1[a1,m] VEC2[a2,n] (mn|rs) VEC1[b1,r] VEC2[b2,s]."""
+    sym, xyz = G.water()
+    om = O.OracleMolecule.from_angstrom(sym, xyz, os.path.join(BASIS, "cc-pvdz.gbs"),
+                                        os.path.join(BASIS, "cc-pvdz-jkfit.gbs"))
+    eri3, V = om.eri3_nopt(), om.metric()
+    n = eri3.shape[0]
+    rng = np.random.default_rng(3)
+    V1, V2 = rng.standard_normal((4, n)), rng.standard_normal((4, n))
+    got = O.nopt_mo_ints(eri3, O.s05(V)[0], V1, V2)
+    W = np.linalg.solve(np.linalg.cholesky(V), eri3.reshape(n * n, -1).T)
+    g4 = (W.T @ W).reshape(n, n, n, n)
+    ref = np.einsum("am,bn,mnrs,cr,ds->badc", V1, V2, g4, V1, V2, optimize=True)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+    # a transposed orbital set is caught
+    bad = np.einsum("am,bn,mnrs,cr,ds->abdc", V1, V2, g4, V1, V2, optimize=True)
+    assert np.abs(got - bad).max() > 1e-3 * np.abs(ref).max()
