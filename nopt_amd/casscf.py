@@ -158,7 +158,9 @@ class ActiveSpace:
             op = sla.LinearOperator((nd, nd), matvec=lambda x: self.sigma(h, g, x), dtype=np.float64)
             k = nstates + 4
             while True:
-                v0 = guess[:, 0] if guess is not None else None
+                # the last ground state, else a fixed-seed Gaussian vector (deterministic, and
+                # with components along every state, which a uniform vector can lack)
+                v0 = guess[:, 0] if guess is not None else np.random.default_rng(0).standard_normal(nd)
                 w, V = sla.eigsh(op, k=k, which="SA", v0=v0, tol=1e-13)
                 o = np.argsort(w)
                 w, V = w[o], V[:, o]
