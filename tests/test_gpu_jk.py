@@ -204,6 +204,29 @@ def test_nopt_mo_ints_through_the_shim(c2h4_case, t):
     assert np.abs(out2 - ref).max() < 1e-10 * scale
 
 
+def test_active_ints_rejects_bad_requests(c2h4_case):
+    """nopt_jk_active_ints: na outside 1..32 is NOPT_EINVAL and a context
+    without the FP64 form of B is NOPT_ESTATE, each before any device work;
+    the context stays usable afterwards."""
+    from nopt_amd import _lib as L
+    om, r, pm = c2h4_case
+    n = pm.nbf
+    C = np.random.default_rng(1).standard_normal((33, n))
+    with JKContext(n, pm.naux) as ctx:
+        ctx.set_B(pm.eri3_packed(), pm.metric())
+        for bad in (C[:0], C[:33]):
+            with pytest.raises(L.NoptError) as ei:
+                ctx.active_ints(bad)
+            assert ei.value.code == L.NOPT_EINVAL
+        ok = ctx.active_ints(C[:2])
+        assert ok.shape == (2, 2, 2, 2) and np.isfinite(ok).all()
+    with JKContext(n, pm.naux, keep="int8") as ctx:
+        ctx.set_B(pm.eri3_packed(), pm.metric())
+        with pytest.raises(L.NoptError) as ei:
+            ctx.active_ints(C[:2])
+        assert ei.value.code == L.NOPT_ESTATE
+
+
 # (system, orbital file, tolerance on the orbital energies): the file stores
 # energies to NOPT's convergence, and the FP64 build at the file's orbitals
 # differs from them by 5.3e-7 (pBQ) and 2.8e-6 (octatetraene, the oracle's
