@@ -30,6 +30,12 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# the form of B each mode reads (all are formed once per geometry from the
+# same FP64 B = L^-1 (P|mn))
+B_FORM = {"int8": "int8 digit planes (4 planes of the full rows for the half-transform, 5-byte packed "
+                  "lower triangle for gamma and J)",
+          "fp32": "FP64 rows (rounded to FP32 on the fly)", "bf16x6": "FP64 rows (split to bf16 on the fly)",
+          "bf16x3": "FP64 rows (split to bf16 on the fly)", "fp64": "FP64 rows"}
 PEAK = {"hbm_GBps": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3, "bf16_TF": 2500.0, "int8_TOPS": 5000.0}
 
 
@@ -81,10 +87,14 @@ KERNELS = {
 }
 
 
-def roofline(phase, ms, nbf, naux, nocc, prec):
+def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
     """Algorithmic work of the phase's kernel per launch, over its HIP-event time.
-    Bytes count each operand element once at its stored width (no padding);
-    int8 counts 13 digit products per FP64 product."""
+    Flops are the FP64 contraction's (F_half = 2 Q nbf^2 o, F_K = Q o nbf(nbf+1),
+    one triangle), whatever arithmetic carries them, against the peak of the
+    arithmetic used; the int8 modes' digit-pair products (13 per product) are
+    reported beside as the emulation's own rate. Bytes count each operand
+    element once at its stored width (no padding): B in the form the kernel
+    reads (full rows or packed lower triangle), C, and X written."""
     t = ms * 1e-3
     npair = nbf * (nbf + 1) / 2
 
@@ -98,19 +108,32 @@ def roofline(phase, ms, nbf, naux, nocc, prec):
         return dict(bound="mfma", achieved=a, peak=peak, unit=unit, frac=a / peak, traffic=None,
                     kernel=KERNELS[phase][prec], algorithmic_flops=ops)
 
+    f_half = 2.0 * naux * nbf * nbf * nocc
+    f_k = float(naux * nocc * nbf * (nbf + 1))
     if phase == "half_transform":
         if prec == "fp64":
-            return mfma(2.0 * naux * nbf * nbf * nocc, PEAK["fp64_TF"])
+            return mfma(f_half, PEAK["fp64_TF"])
         if prec == "int8":  # 4 B digit planes in, 4 X digit planes out
-            return hbm(4.0 * naux * nbf * nbf + 4.0 * naux * nbf * nocc)
+            r = hbm(4.0 * naux * (npair if half_packed else nbf * nbf) + 4.0 * naux * nbf * nocc)
+            r["b_form"] = "4 int8 digit planes, " + ("packed lower triangle" if half_packed else "full rows")
+            r["mfma_view"] = dict(algorithmic_flops=f_half, achieved_TOPs=f_half / t / 1e12,
+                                  frac_of_int8_peak=f_half / t / 1e12 / PEAK["int8_TOPS"],
+                                  emulation_frac=13 * f_half / t / 1e12 / PEAK["int8_TOPS"])
+            return r
         return hbm(8.0 * naux * nbf * nbf + 4.0 * naux * nbf * nocc)  # FP64 B in, FP32 X out
     if phase == "exchange":
-        f = float(naux * nocc * nbf * (nbf + 1))
         if prec == "int8":
-            return mfma(13 * f, PEAK["int8_TOPS"], "TOP/s")
+            r = mfma(f_k, PEAK["int8_TOPS"], "TOP/s")
+            # the 13 digit-pair products per product on the int8 matrix cores,
+            # and the rate as the FP64 work it replaces
+            r["emulation"] = dict(int8_ops=13 * f_k, achieved_TOPs=13 * f_k / t / 1e12,
+                                  frac=13 * f_k / t / 1e12 / PEAK["int8_TOPS"])
+            r["fp64_equivalent"] = dict(achieved_TFLOPs=f_k / t / 1e12,
+                                        frac_of_fp64_peak=f_k / t / 1e12 / PEAK["fp64_TF"])
+            return r
         pk = {"fp64": PEAK["fp64_TF"], "fp32": PEAK["fp32_TF"], "bf16x6": PEAK["bf16_TF"] / 6,
               "bf16x3": PEAK["bf16_TF"] / 3}[prec]
-        return mfma(f, pk)
+        return mfma(f_k, pk)
     if phase == "gamma":
         if prec == "int8":  # five digit planes of the lower triangle
             return hbm(5.0 * naux * npair)
@@ -159,37 +182,18 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
     graph and of the same builds launched one by one, with the split counts
     of each system set by the autotuner (the replay with the counts picked at
     context creation is reported beside it)."""
-    from nopt_amd import geometry as G
-    from nopt_amd.molecule import Molecule
     from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
-    sizes = G.batch_sizes(count, nmax, seed=seed)
+    from nopt_amd.workloads import ragged_clusters
     t0 = time.perf_counter()
-    systems = []
-    for k, n in enumerate(sizes):
-        sym, xyz = G.water_cluster(n, seed=seed + 1 + k)
-        m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit")
-        S, T, V = m.one_electron()
-        s_, U = np.linalg.eigh(S)
-        X = U @ np.diag(s_ ** -0.5) @ U.T
-        C = X @ np.linalg.eigh(X.T @ (T + V) @ X)[1]
-        ncore = m.nocc - 5
-        rng = np.random.default_rng(seed + 1 + k)
-        A = rng.standard_normal((10, 10))
-        gam = A @ A.T
-        gam *= 10.0 / np.trace(gam)
-        occ, W = np.linalg.eigh(gam)
-        occ = np.clip(occ, 0.0, None)
-        Cc = np.ascontiguousarray(C[:, :ncore].T)
-        NO = W.T @ C[:, ncore:ncore + 10].T
-        Ct = np.ascontiguousarray(np.vstack([np.sqrt(2.0) * Cc, NO * np.sqrt(0.5 * occ)[:, None]]))
-        Dc = Cc.T @ Cc
-        Dt = Ct.T @ Ct
+    systems, sizes = [], []
+    for n, m, (Dc, Cc, Dt, Ct) in ragged_clusters(count, nmax, seed):
         ctx = JKContext(m.nbf, m.naux)
         ctx.set_B(m.eri3_packed(), m.metric())
-        d = dict(m=m, ctx=ctx, ncore=ncore, nt=Ct.shape[0], host=(Dc, Cc, Dt, Ct))
+        d = dict(m=m, ctx=ctx, ncore=Cc.shape[0], nt=Ct.shape[0], host=(Dc, Cc, Dt, Ct))
         d["dev"] = [DeviceArray.from_host(a) if a.size else DeviceArray((1,)) for a in (Dc, Cc, Dt, Ct)]
         d["out"] = [DeviceArray((m.nbf, m.nbf)) for _ in range(4)]
         systems.append(d)
+        sizes.append(n)
     setup = time.perf_counter() - t0
     stream = systems[0]["ctx"].stream()
 
@@ -377,11 +381,49 @@ def large_system(n_water, seed, steps, threads):
     }
 
 
-def cpu_baseline(sym, xyz_ang, threads, nbuilds=5, label="C20H42"):
+def host_cores():
+    """Physical cores this process may run on: the distinct (package, core)
+    pairs of the CPUs in its affinity mask (sysfs topology), capped by the
+    host's CPU share for this job (OMP_NUM_THREADS when the environment sets
+    it). Returns (cores, description)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            phys.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            phys.add(("?", str(c)))
+    n = len(phys)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(share) if share.isdigit() and int(share) > 0 else None
+    cores = min(n, cap) if cap else n
+    desc = (f"{n} physical cores ({len(cpus)} logical CPUs) in the affinity mask"
+            + (f", CPU share OMP_NUM_THREADS={cap}" if cap else "") + f": {cores} threads")
+    return cores, desc
+
+
+def converged_cartesian(sym, xyz_ang):
+    """The converged FP64 RI-RHF orbitals of the workload in NOPT's Cartesian
+    representation (the basis of the CPU path), from GPU builds: (DM, MO) with
+    MO rows the occupied orbitals."""
+    from nopt_amd.molecule import Molecule
+    from nopt_amd.jk import JKContext
+    from nopt_amd.scf import rhf
+    mc = Molecule.from_angstrom(sym, xyz_ang, pure=False)
+    with JKContext(mc.nbf, mc.naux) as ctx:
+        ctx.set_B(mc.eri3_packed(), mc.metric())
+        r = rhf(mc, ctx=ctx, prec="fp64", e_conv=1e-9, d_conv=1e-7)
+    MO = np.ascontiguousarray(r.C[:, :mc.nocc].T)
+    return MO.T @ MO, MO, r.E
+
+
+def cpu_baseline(sym, xyz_ang, threads, DM, MO, nbuilds=5, label="C20H42", cores_desc=""):
     """NOPT's CPU path (oracle restatement) on a bounded sample: one warm-up
     and `nbuilds` timed builds of the full C20H42 Cartesian workload (V^-1/2
-    applied per build, as DM_to_F_transform_RI does), then the same for the
-    "B-prepared" variant, whose stored tensor already holds the metric."""
+    applied per build, as DM_to_F_transform_RI does) at the converged density
+    and occupied orbitals (DM, MO), then the same for the "B-prepared"
+    variant, whose stored tensor already holds the metric."""
     os.environ["ORACLE_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -393,10 +435,7 @@ def cpu_baseline(sym, xyz_ang, threads, nbuilds=5, label="C20H42"):
     RI_AA = mc.eri3_nopt()
     Vm05, _, _ = O.s05(mc.metric())
     O.lib().oracle_set_threads(threads)
-    rng = np.random.default_rng(0)
-    MO, _ = np.linalg.qr(rng.standard_normal((mc.nbf, mc.nbf)))
-    MO = np.ascontiguousarray(MO.T)
-    DM = MO[:mc.nocc].T @ MO[:mc.nocc]
+    assert MO.shape == (mc.nocc, mc.nbf) and DM.shape == (mc.nbf, mc.nbf)
     setup = time.perf_counter() - t0
 
     def median_s(jk):
@@ -420,7 +459,9 @@ def cpu_baseline(sym, xyz_ang, threads, nbuilds=5, label="C20H42"):
     return dict(value=1.0 / med, unit="builds/s", cores=threads, kind="port",
                 sample=f"{nbuilds} timed builds after 1 warm-up of the oracle's C restatement of NOPT's "
                        f"DM_to_F_transform_RI (MKL DGEMM, OpenMP row split as nopt_par_dgemm) on {label}: "
-                       f"Cartesian n_ao {mc.nbf}, naux {mc.naux}, n_c {mc.nocc}; median {med:.3f} s/build",
+                       f"Cartesian n_ao {mc.nbf}, naux {mc.naux}, n_c {mc.nocc}, converged FP64 RI-RHF "
+                       f"density and orbitals; median {med:.3f} s/build",
+                cores_detail=cores_desc, blas="MKL (libmkl_rt)",
                 b_prepared=dict(value=1.0 / medp, unit="builds/s", median_s=round(medp, 4), prepare_s=round(prep_s, 1),
                                 sample=f"{nbuilds} timed builds after 1 warm-up, same workload with RI_AA.V^-1/2 "
                                        f"formed once (no metric products per build)"),
@@ -442,7 +483,8 @@ def main():
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged water-cluster batch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (default: the physical cores of this job's CPU share)")
     ap.add_argument("--large", type=int, default=0, metavar="NWATER",
                     help="instead of the headline, run the large-system RI-RHF on (H2O)_NWATER (seed 1) and exit")
     args = ap.parse_args()
@@ -462,6 +504,11 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.nbytes, "copies": 5}))
         return
 
+    cores, cores_desc = host_cores()
+    if args.cpu_threads <= 0:
+        args.cpu_threads = cores
+    else:
+        cores_desc = f"{args.cpu_threads} threads (--cpu-threads); " + cores_desc
     if args.large:
         print(json.dumps(large_system(args.large, 1, max(3, min(args.steps, 10)), args.cpu_threads)))
         return
@@ -563,7 +610,10 @@ def main():
         for a in (dD, dC, dJ, dK):
             a.free()
         t0 = time.perf_counter()
-        cpu = cpu_baseline(sym, xyz, args.cpu_threads, label=f"C{args.alkane}H{2 * args.alkane + 2}")
+        DMc, MOc, Ec = converged_cartesian(sym, xyz)
+        log(f"[bench] Cartesian FP64 RI-RHF for the CPU path: E = {Ec:.10f} ({time.perf_counter() - t0:.1f} s)")
+        cpu = cpu_baseline(sym, xyz, args.cpu_threads, DMc, MOc, label=f"C{args.alkane}H{2 * args.alkane + 2}",
+                           cores_desc=cores_desc)
         log(f"[bench] CPU baseline {cpu['value']:.3f} builds/s on {cpu['cores']} threads, B-prepared variant "
             f"{cpu['b_prepared']['value']:.3f} builds/s ({time.perf_counter() - t0:.1f} s)")
 
@@ -582,7 +632,7 @@ def main():
         "data": "C20H42 geometry built in-repo; converged FP64 RI-RHF density and orbitals",
         "config": {
             "workload": f"RI-J/K build, all-trans C{args.alkane}H{2 * args.alkane + 2} cc-pVDZ/cc-pVDZ-JKFIT, "
-                        f"nbf {nbf}, naux {naux}, nocc {nocc}, B FP64 resident in HBM",
+                        f"nbf {nbf}, naux {naux}, nocc {nocc}, B resident in HBM as {B_FORM[prec]}",
             "precision": prec,
             "arithmetic": {"int8": "int8 digit products, exact int32 class sums, fp64 combine; J from 35-bit "
                                    "integers in fp64",

@@ -19,6 +19,11 @@
  *    (src/RI.cpp:36-39, std::bad_alloc); here they are reported.
  *  - Calls are host-synchronous unless they take a stream; one context must
  *    not be used from two threads at once.
+ *  - Work enqueued on one context runs in call order whatever the streams:
+ *    every call waits, on its own stream, for the context's previous work
+ *    (an event recorded at the end of each call), because all builds of a
+ *    context share its work buffers. Builds of different contexts on
+ *    different streams may overlap.
  */
 #ifndef NOPT_JK_H
 #define NOPT_JK_H
@@ -184,7 +189,14 @@ void* nopt_jk_stream(nopt_jk_ctx* ctx);
  * are recorded instead of run; launch replays them all with one call. The
  * inputs and outputs are the device buffers named at capture time (update
  * their contents between launches). Run one build of each shape before
- * capturing: buffers cannot grow during a capture (NOPT_ESTATE). */
+ * capturing: buffers cannot grow during a capture (NOPT_ESTATE).
+ * A graph holds the addresses of the work buffers of every context it
+ * recorded, so from capture_end until nopt_graph_destroy those contexts are
+ * pinned: a build that would grow a buffer (more occupied orbitals, a larger
+ * batch, another mode), nopt_jk_set_tuning and nopt_jk_autotune fail with
+ * NOPT_ESTATE, and nopt_jk_destroy only marks the context, which is freed
+ * when its last graph is destroyed. A launch waits for the earlier work of
+ * the recorded contexts and later calls on them wait for the launch. */
 typedef struct nopt_graph nopt_graph;
 int nopt_graph_capture_begin(int device, void* stream);
 int nopt_graph_capture_end(int device, void* stream, nopt_graph** out);

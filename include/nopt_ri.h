@@ -2,8 +2,9 @@
 // (include/RI.h:6-23), backed by the GPU context of nopt_jk.h.
 //
 // The functions keep NOPT's names, argument lists and semantics, with C linkage
-// (NOPT declares them as C++ functions), so an SCF or CASSCF loop written
-// against NOPT's API compiles against this header and links to libnoptjk.so:
+// here; the build functions are exported with NOPT's C++ linkage as well
+// (include/RI.h:10-16), so an SCF or CASSCF loop written against NOPT's API
+// compiles against either header and links to libnoptjk.so:
 //   - D is normalised to 1 and MO rows are orbitals; K uses the first n_c rows
 //     (DM_to_F_transform_RI, src/RI.cpp:115-176); J and K are overwritten.
 //   - DM_to_F_transform_RI returns 1 on success like NOPT's; the others return
@@ -38,13 +39,20 @@ int RI_gpu_attach_molecule(int natom, const int* Z, const double* xyz_bohr, cons
 // (a NOPT_PREC_* value of nopt_jk.h; FP64 by default).
 int RI_gpu_set_precision(int prec);
 
-// NOPT's RI_core_realloc (src/RI.cpp:104-113): the device buffers grow on
-// demand, so this only checks that a context exists.
-int RI_core_realloc(long n_c, long n_ao);
-
 // NOPT's regen_RI_AA (src/RI.cpp:81-101) minus the rebuild: releases the
 // device context so that the next RI_gpu_attach forms B for a new geometry.
 int regen_RI_AA_release();
+
+// The three build functions below are also exported with C++ linkage under
+// the signatures of NOPT's include/RI.h:10-16, which NOPT's own callers link
+// to. A translation unit that also includes RI.h (NOPT's src/RI.cpp) defines
+// NOPT_RI_SETUP_ONLY before including this header: it then sees only the
+// set-up functions above, since a C and a C++ declaration of one function
+// cannot coexist.
+#ifndef NOPT_RI_SETUP_ONLY
+// NOPT's RI_core_realloc (src/RI.cpp:104-113): the device buffers grow on
+// demand, so this only checks that a context exists.
+int RI_core_realloc(long n_c, long n_ao);
 
 // NOPT's DM_to_F_transform_RI (src/RI.cpp:115-176).
 int DM_to_F_transform_RI(double* J, double* K, double* DM, double* MO, long n_c, long n_ao);
@@ -57,6 +65,7 @@ int DM_to_F_transform_RI(double* J, double* K, double* DM, double* MO, long n_c,
 // like NOPT), J from DM_D and K from the first n_c rows of MO_VEC.
 int calc_2el_MO_INTS_RI(long n_ao, double* DM_D, double* J, double* K, double* MO_INTS, double* MO_VEC,
                         double* MO_VEC1, double* MO_VEC2, long n_c, long n_a, long t);
+#endif  // NOPT_RI_SETUP_ONLY
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,12 @@ class ActiveSpace:
         asking for more roots until enough have the requested spin."""
         nd = len(self.dets)
         S = 0.5 * (mult - 1)
+        # a spin the determinants cannot carry: S >= |Ms|, S at most half the
+        # number of singly occupiable orbitals, 2S of the parity of the electrons
+        s_max = 0.5 * min(self.nel, 2 * self.n - self.nel)
+        if S < abs(self.ms) - 1e-9 or S > s_max + 1e-9 or (mult - 1 - self.nel) % 2:
+            raise ValueError(f"no states of multiplicity {mult} for {self.nel} electrons in {self.n} orbitals "
+                             f"with Ms = {self.ms}")
         if nd <= 400:
             w, V = np.linalg.eigh(self.hamiltonian(h, g))
         else:
@@ -165,6 +171,8 @@ class ActiveSpace:
                 o = np.argsort(w)
                 w, V = w[o], V[:, o]
                 if sum(abs(self.spin_squared(V[:, j]) - S * (S + 1)) < 1e-6 for j in range(k)) >= nstates:
+                    break
+                if k >= nd - 1:  # every root eigsh can give: the spin filter left too few
                     break
                 k = min(2 * k, nd - 1)
         pick = [j for j in range(len(w)) if abs(self.spin_squared(V[:, j]) - S * (S + 1)) < 1e-6][:nstates]

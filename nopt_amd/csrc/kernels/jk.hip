@@ -647,6 +647,15 @@ struct nopt_jk_ctx {
   bool timing = false;
   int timed = 0;
   std::vector<hipEvent_t> tev;
+  // completion of the last work enqueued on the context's buffers, on
+  // whatever stream it ran: the next call waits for it on its own stream
+  hipEvent_t ev_done = nullptr;
+  bool done_set = false;
+  // HIP graphs that hold the addresses of the work buffers: while any exists
+  // the buffers keep their addresses (no growth, no tuning change) and
+  // destroy is deferred to the last graph's destruction
+  int pins = 0;
+  bool zombie = false;
 };
 
 namespace {
@@ -694,13 +703,19 @@ void dev_free(P*& p) {
 // set between nopt_graph_capture_begin and _end on this thread: device buffers
 // may not grow while a stream is being captured
 thread_local bool t_capturing = false;
+// contexts whose builds were recorded by the capture in progress on this thread
+thread_local std::vector<nopt_jk_ctx*> t_captured;
 
-int ensure(void** p, size_t* cap, size_t bytes, const char* what) {
+int ensure(const nopt_jk_ctx* c, void** p, size_t* cap, size_t bytes, const char* what) {
   if (*cap >= bytes && *p) return NOPT_OK;
   if (t_capturing)
     return set_error(NOPT_ESTATE, std::string("the ") + what +
                                       " buffer would grow during graph capture; run one build of the same shape "
                                       "before capturing");
+  if (c->pins > 0)
+    return set_error(NOPT_ESTATE, std::string("the ") + what +
+                                      " buffer would grow while a captured HIP graph refers to it; destroy the "
+                                      "graph first");
   if (*p) (void)hipFree(*p);
   *p = nullptr;
   *cap = 0;
@@ -758,6 +773,29 @@ int check_ctx(nopt_jk_ctx* c, bool need_B) {
   if (need_B && !c->b_ready) return set_error(NOPT_ESTATE, "B tensor not set (call nopt_jk_set_B first)");
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return set_error(NOPT_EHIP, std::string("hipSetDevice: ") + hip_msg(e));
+  return NOPT_OK;
+}
+
+// Every call that enqueues work on a context's buffers waits, on its own
+// stream, for the last work enqueued on them (order_in) and records its own
+// completion (order_out), so that builds on different streams (the default
+// stream, torch's stream, the context's own) run one after the other instead
+// of sharing the work buffers. Nothing is recorded or awaited while a capture
+// is in progress on this thread; the capture notes the context instead, and
+// the graph's launch does the ordering.
+int order_in(nopt_jk_ctx* c, hipStream_t s) {
+  if (t_capturing || !c->done_set) return NOPT_OK;
+  HIPCHK(hipStreamWaitEvent(s, c->ev_done, 0));
+  return NOPT_OK;
+}
+
+int order_out(nopt_jk_ctx* c, hipStream_t s) {
+  if (t_capturing) {
+    if (std::find(t_captured.begin(), t_captured.end(), c) == t_captured.end()) t_captured.push_back(c);
+    return NOPT_OK;
+  }
+  HIPCHK(hipEventRecord(c->ev_done, s));
+  c->done_set = true;
   return NOPT_OK;
 }
 
@@ -846,13 +884,113 @@ void stage_rows(const double* src, long lds, int rows, int cols, T* dst, long ld
   if (n > 0) k_pad_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, lds, rows, cols, dst, ldd, rows_pad);
 }
 
+// Work buffers of one build: the context's own (grown on demand) or carved
+// out of a caller-independent scratch area (the allocation-free entry point).
+struct WorkBytes {
+  size_t X = 0, Cp = 0, Cd = 0, Ce = 0, Xd = 0, Xe = 0, Jpart = 0, Kpart = 0, Dp = 0, gamma = 0, gpart = 0,
+         gslot = 0;
+};
+struct Work {
+  void *X = nullptr, *Cp = nullptr, *Cd = nullptr, *Ce = nullptr, *Xd = nullptr, *Xe = nullptr, *Jpart = nullptr;
+  double *Kpart = nullptr, *Dp = nullptr, *gamma = nullptr, *gpart = nullptr, *gslot = nullptr;
+};
+
+// The context's buffers, grown to the sizes of this build (the fixed-size
+// ones were allocated by create and cover every build).
+int context_work(nopt_jk_ctx* c, const WorkBytes& b, Work* w) {
+  int rc = NOPT_OK;
+  if (b.X) {
+    size_t cap = c->X_bytes;
+    rc = ensure(c, &c->X, &cap, b.X, "half-transformed tensor X");
+    if (rc) return rc;
+    c->X_bytes = cap;
+  }
+  if (b.Cp) {
+    size_t cap = c->Cp_bytes;
+    rc = ensure(c, &c->Cp, &cap, b.Cp, "padded orbital factor");
+    if (rc) return rc;
+    c->Cp_bytes = cap;
+  }
+  if (b.Cd) {
+    size_t b1 = c->Cd_bytes, b2 = c->Ce_bytes;
+    rc = ensure(c, &c->Cd, &b1, b.Cd, "orbital digit planes");
+    if (!rc) rc = ensure(c, &c->Ce, &b2, b.Ce, "orbital exponents");
+    if (rc) return rc;
+    c->Cd_bytes = b1;
+    c->Ce_bytes = b2;
+  }
+  if (b.Xd) {
+    size_t b3 = c->Xd_bytes, b4 = c->Xe_bytes;
+    rc = ensure(c, &c->Xd, &b3, b.Xd, "exchange digit planes");
+    if (!rc) rc = ensure(c, &c->Xe, &b4, b.Xe, "exchange row exponents");
+    if (rc) return rc;
+    c->Xd_bytes = b3;
+    c->Xe_bytes = b4;
+  }
+  {
+    size_t kb = c->Kpart_bytes;
+    rc = ensure(c, (void**)&c->Kpart, &kb, b.Kpart, "K partials");
+    if (rc) return rc;
+    c->Kpart_bytes = kb;
+  }
+  if (b.Jpart) {
+    size_t jb = c->Jpart_bytes;
+    rc = ensure(c, &c->Jpart, &jb, b.Jpart, "J partials");
+    if (rc) return rc;
+    c->Jpart_bytes = jb;
+  }
+  w->X = c->X;
+  w->Cp = c->Cp;
+  w->Cd = c->Cd;
+  w->Ce = c->Ce;
+  w->Xd = c->Xd;
+  w->Xe = c->Xe;
+  w->Jpart = c->Jpart;
+  w->Kpart = c->Kpart;
+  w->Dp = c->Dp;
+  w->gamma = c->gamma;
+  w->gpart = c->gpart;
+  w->gslot = c->gslot;
+  return NOPT_OK;
+}
+
+// The same buffers as consecutive 256-byte aligned pieces of `base` (bytes
+// available: `cap`); nothing is allocated.
+int carve_work(void* base, size_t cap, const WorkBytes& b, Work* w) {
+  char* p = (char*)base;
+  size_t used = 0;
+  auto take = [&](size_t bytes) -> void* {
+    if (!bytes) return nullptr;
+    void* r = p + used;
+    used += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  w->X = take(b.X);
+  w->Cp = take(b.Cp);
+  w->Cd = take(b.Cd);
+  w->Ce = take(b.Ce);
+  w->Xd = take(b.Xd);
+  w->Xe = take(b.Xe);
+  w->Jpart = take(b.Jpart);
+  w->Kpart = (double*)take(b.Kpart);
+  w->Dp = (double*)take(b.Dp);
+  w->gamma = (double*)take(b.gamma);
+  w->gpart = (double*)take(b.gpart);
+  w->gslot = (double*)take(b.gslot);
+  if (used > cap)
+    return set_error(NOPT_EINVAL, "this build needs " + std::to_string(used) + " bytes of work space; the static "
+                                  "scratch area holds " + std::to_string(cap) + " (use nopt_jk_build_device)");
+  return NOPT_OK;
+}
+
 // Enqueues the J/K builds of nd densities that share B. The orbital factors
 // are stacked into one padded factor, so the half-transform reads B (or its
 // digit planes) once for all of them; gamma and J are formed for every
 // density in one pass over B each; K is one triangular product per density.
 // Phases (for the optional timing): staging, half-transform, gamma, J
 // partials, exchange (K products and K output), J output.
-int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStream_t s) {
+int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStream_t s, void* work_area = nullptr,
+                  size_t work_area_bytes = 0) {
   const int nbf = (int)c->nbf, naux = (int)c->naux;
   if (prec < 0 || prec > 4) return set_error(NOPT_EINVAL, "unknown precision mode");
   if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
@@ -901,57 +1039,37 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   xoff[0] = 0;
   for (int d = 0; d < nd; ++d)
     xoff[d + 1] = xoff[d] + ((off[d + 1] > off[d]) ? 4 * (xfused ? rowsq : rows) * ldk_of(it[d].nocc) : 0);
+  // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
+  const int nsj_min = !nj ? 0 : (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj);
+  const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(nsj_min, c->nsj > 0 ? c->nsj : NSPLIT_J_MIN);
+  // bytes of each work buffer this build uses
+  WorkBytes wb;
   if (anyK) {
-    if (!xfused) {
-      size_t cap = c->X_bytes;
-      rc = ensure(&c->X, &cap, esz * rows * ldx, "half-transformed tensor X");
-      if (rc) return rc;
-      c->X_bytes = cap;
-    }
-    size_t cpb = c->Cp_bytes;
-    rc = ensure(&c->Cp, &cpb, esz * ldx * ldb, "padded orbital factor");
-    if (rc) return rc;
-    c->Cp_bytes = cpb;
+    if (!xfused) wb.X = esz * rows * ldx;
+    wb.Cp = esz * ldx * ldb;
     if (i8) {
-      size_t b1 = c->Cd_bytes, b2 = c->Ce_bytes;
-      rc = ensure(&c->Cd, &b1, (size_t)4 * ldx * c->ld8, "orbital digit planes");
-      if (!rc) rc = ensure(&c->Ce, &b2, sizeof(int) * ldx, "orbital exponents");
-      if (rc) return rc;
-      c->Cd_bytes = b1;
-      c->Ce_bytes = b2;
+      wb.Cd = (size_t)4 * ldx * c->ld8;
+      wb.Ce = sizeof(int) * ldx;
       long ldk_max = 0;
       for (int d = 0; d < nd; ++d)
         if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, ldk_of(it[d].nocc));
       // fused: planes of every density side by side; otherwise one density
       // at a time
-      const size_t xdb = xfused ? (size_t)xoff[nd] : (size_t)4 * rows * ldk_max;
-      const size_t xeb = sizeof(int) * (xfused ? rowsq * nd : rows);
-      size_t b3 = c->Xd_bytes, b4 = c->Xe_bytes;
-      rc = ensure(&c->Xd, &b3, xdb, "exchange digit planes");
-      if (!rc) rc = ensure(&c->Xe, &b4, xeb, "exchange row exponents");
-      if (rc) return rc;
-      c->Xd_bytes = b3;
-      c->Xe_bytes = b4;
+      wb.Xd = xfused ? (size_t)xoff[nd] : (size_t)4 * rows * ldk_max;
+      wb.Xe = sizeof(int) * (xfused ? rowsq * nd : rows);
     }
   }
-  // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
-  const int nsj_min = !nj ? 0 : (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj);
-  const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(nsj_min, c->nsj > 0 ? c->nsj : NSPLIT_J_MIN);
-  {  // K partials of the current split counts
-    size_t kb = c->Kpart_bytes;
-    const size_t need = sizeof(double) * (i8 ? (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
-                                              : (size_t)c->nsk * c->ntiles * KT * KT);
-    rc = ensure((void**)&c->Kpart, &kb, need, "K partials");
-    if (rc) return rc;
-    c->Kpart_bytes = kb;
-  }
-  if (nj) {
-    size_t jb = c->Jpart_bytes;
-    rc = ensure(&c->Jpart, &jb, sizeof(double) * nj * nsj * nn, "J partials");
-    if (rc) return rc;
-    c->Jpart_bytes = jb;
-  }
-  double* Jpart = (double*)c->Jpart;
+  wb.Kpart = sizeof(double) * (i8 ? (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
+                                  : (size_t)c->nsk * c->ntiles * KT * KT);
+  if (nj) wb.Jpart = sizeof(double) * nj * nsj * nn;
+  wb.Dp = sizeof(double) * std::max(nj, 1) * c->dstride;
+  wb.gamma = sizeof(double) * std::max(nj, 1) * naux;
+  wb.gpart = sizeof(double) * naux * c->mtiles64;
+  wb.gslot = sizeof(double) * nopt::gamma_partial_blocks(c->nslots, c->nseg) * std::max(nj, 1) * naux;
+  Work w;
+  rc = work_area ? carve_work(work_area, work_area_bytes, wb, &w) : context_work(c, wb, &w);
+  if (rc) return rc;
+  double* Jpart = (double*)w.Jpart;
   const bool tm = c->timing && c->timed < kTimedBuilds;
   auto mark = [&](int ph) {
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
@@ -967,14 +1085,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   };
   auto coulomb_gamma = [&]() -> hipError_t {
     if (fused) {
-      k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(c->gpart, c->mtiles, naux, c->gamma);
+      k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(w.gpart, c->mtiles, naux, w.gamma);
     } else if (i8) {
       CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                    c->nslots, c->segs, c->nseg, c->Dp, ldb, c->dstride, c->gslot, c->gamma, naux));
+                                    c->nslots, c->segs, c->nseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux));
     } else {
 #define NOPT_GF(ND)                                                                                         \
   case ND:                                                                                                  \
-    k_gamma_f64<ND><<<naux, 256, 0, sj>>>(c->B64, c->ld64, nbf, c->Dp, ldb, c->dstride, c->gamma, naux); \
+    k_gamma_f64<ND><<<naux, 256, 0, sj>>>(c->B64, c->ld64, nbf, w.Dp, ldb, c->dstride, w.gamma, naux); \
     break;
       switch (nj) { NOPT_GF(1) NOPT_GF(2) NOPT_GF(3) NOPT_GF(4) }
 #undef NOPT_GF
@@ -986,11 +1104,11 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   auto coulomb_j = [&]() -> hipError_t {
     if (i8)
       return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                      c->nslots, c->segs, c->nseg, c->gamma, naux, Jpart, (long)nsj * nn);
+                                      c->nslots, c->segs, c->nseg, w.gamma, naux, Jpart, (long)nsj * nn);
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                     \
   case ND:                                                                                              \
-    k_jpart<ND><<<gj, 256, 0, sj>>>(c->B64, c->ld64, nbf, naux, c->gamma, naux, Jpart, (long)nsj * nn); \
+    k_jpart<ND><<<gj, 256, 0, sj>>>(c->B64, c->ld64, nbf, naux, w.gamma, naux, Jpart, (long)nsj * nn); \
     break;
     switch (nj) { NOPT_JP(1) NOPT_JP(2) NOPT_JP(3) NOPT_JP(4) }
 #undef NOPT_JP
@@ -1011,28 +1129,28 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   mark(0);
   // densities of the J list into slots 0..nj-1 (ld = ldb); orbital blocks
   // into the stacked factor at column offsets off[d] (zero rows between)
-  for (int k = 0; k < nj; ++k) stage_rows<double>(it[jl[k]].D, nbf, nbf, nbf, c->Dp + k * c->dstride, ldb, nbf, s);
+  for (int k = 0; k < nj; ++k) stage_rows<double>(it[jl[k]].D, nbf, nbf, nbf, w.Dp + k * c->dstride, ldb, nbf, s);
   if (anyK) {
     for (int d = 0; d < nd; ++d) {
       const int pad = (int)(off[d + 1] - off[d]);
       if (!pad) continue;
       const int r = (int)it[d].nocc;
       if (i8)
-        HIPCHK(nopt::split_rows_i8(it[d].C, nbf, nbf, r, 4, (int8_t*)c->Cd + off[d] * c->ld8, c->ld8, pad,
-                                   ldx * c->ld8, (int*)c->Ce + off[d], s));
+        HIPCHK(nopt::split_rows_i8(it[d].C, nbf, nbf, r, 4, (int8_t*)w.Cd + off[d] * c->ld8, c->ld8, pad,
+                                   ldx * c->ld8, (int*)w.Ce + off[d], s));
       else if (f64)
-        stage_rows<double>(it[d].C, nbf, r, nbf, (double*)c->Cp + off[d] * ldb, ldb, pad, s);
+        stage_rows<double>(it[d].C, nbf, r, nbf, (double*)w.Cp + off[d] * ldb, ldb, pad, s);
       else
-        stage_rows<float>(it[d].C, nbf, r, nbf, (float*)c->Cp + off[d] * ldb, ldb, pad, s);
+        stage_rows<float>(it[d].C, nbf, r, nbf, (float*)w.Cp + off[d] * ldb, ldb, pad, s);
     }
     const int tail = (int)(ldx - ncol);
     if (i8)
-      HIPCHK(nopt::split_rows_i8(c->Dp, ldb, 0, 0, 4, (int8_t*)c->Cd + ncol * c->ld8, c->ld8, tail, ldx * c->ld8,
-                                 (int*)c->Ce + ncol, s));
+      HIPCHK(nopt::split_rows_i8(w.Dp, ldb, 0, 0, 4, (int8_t*)w.Cd + ncol * c->ld8, c->ld8, tail, ldx * c->ld8,
+                                 (int*)w.Ce + ncol, s));
     else if (f64)
-      stage_rows<double>(c->Dp, ldb, 0, 0, (double*)c->Cp + ncol * ldb, ldb, tail, s);
+      stage_rows<double>(w.Dp, ldb, 0, 0, (double*)w.Cp + ncol * ldb, ldb, tail, s);
     else
-      stage_rows<float>(c->Dp, ldb, 0, 0, (float*)c->Cp + ncol * ldb, ldb, tail, s);
+      stage_rows<float>(w.Dp, ldb, 0, 0, (float*)w.Cp + ncol * ldb, ldb, tail, s);
   }
   HIPCHK(hipGetLastError());
   // the integer Coulomb pair starts after the half-transform (it then shares
@@ -1060,23 +1178,23 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
           xs.len[k] = (int)it[d].nocc;
           xs.ldk[k] = ldk;
           xs.xplane[k] = rowsq * ldk;
-          xs.xd[k] = (int8_t*)c->Xd + xoff[d];
-          xs.xe[k] = (int*)c->Xe + (long)d * rowsq;
+          xs.xd[k] = (int8_t*)w.Xd + xoff[d];
+          xs.xe[k] = (int*)w.Xe + (long)d * rowsq;
         }
       if (xfused) gh.y = (unsigned)npq;
       e = nopt::launch_half_i8(nt, xfused ? 2 : 1, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux,
-                               (const int8_t*)c->Cd, ldx * c->ld8, (const int*)c->Ce,
-                               xfused ? nullptr : (double*)c->X, ldx, xs);
+                               (const int8_t*)w.Cd, ldx * c->ld8, (const int*)w.Ce,
+                               xfused ? nullptr : (double*)w.X, ldx, xs);
     }
     else if (f64)
-      e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)c->Cp, c->Dp, (double*)c->X, ldx,
-                              c->gpart, c->mtiles);
+      e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)w.Cp, w.Dp, (double*)w.X, ldx,
+                              w.gpart, c->mtiles);
     else if (prec == NOPT_PREC_FP32)
-      e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)c->Cp, c->Dp, (float*)c->X, ldx,
-                             c->gpart, c->mtiles);
+      e = launch_half<float>(nt, gh, s, c->B64, ldb, nbf, (const float*)w.Cp, w.Dp, (float*)w.X, ldx,
+                             w.gpart, c->mtiles);
     else
       e = nopt::launch_half_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, nt, gh, s, c->B64, ldb, nbf,
-                                 (const float*)c->Cp, c->Dp, (float*)c->X, ldx, c->gpart, c->mtiles);
+                                 (const float*)w.Cp, w.Dp, (float*)w.X, ldx, w.gpart, c->mtiles);
     HIPCHK(e);
   }
   mark(2);
@@ -1093,33 +1211,33 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (!it[d].K) continue;
     const int kpad = (int)(off[d + 1] - off[d]);
     if (!kpad) {  // no occupied orbitals: K = 0
-      stage_rows<double>(c->Dp, nbf, 0, 0, it[d].K, nbf, nbf, s);
+      stage_rows<double>(w.Dp, nbf, 0, 0, it[d].K, nbf, nbf, s);
       continue;
     }
     const unsigned nel = (unsigned)((nn + 255) / 256);
     if (i8) {
       const long ldk = ldk_of(it[d].nocc);
-      const int8_t* xd = (const int8_t*)c->Xd + (xfused ? xoff[d] : 0);
-      const int* xe = (const int*)c->Xe + (xfused ? (long)d * rowsq : 0);
+      const int8_t* xd = (const int8_t*)w.Xd + (xfused ? xoff[d] : 0);
+      const int* xe = (const int*)w.Xe + (xfused ? (long)d * rowsq : 0);
       if (!xfused)
-        HIPCHK(nopt::split_rows_i8((const double*)c->X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
+        HIPCHK(nopt::split_rows_i8((const double*)w.X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
                                    rows, rows * ldk, (int*)xe, s));
       // the exchange runs over the rows of the digit planes: pairs of
       // functions (npq of them) when fused, single functions otherwise
       const long nq = xfused ? npq : naux;
-      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, nq * nbf * ldk, xe, ldk, nbf, (int)nq, c->Kpart));
-      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
+      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, nq * nbf * ldk, xe, ldk, nbf, (int)nq, w.Kpart));
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
                                      it[d].K);
     } else {
       dim3 gk(c->ntiles, c->nsk);
       if (f64)
-        k_syrk<double><<<gk, 256, 0, s>>>((const double*)c->X + off[d], ldx, nbf, naux, kpad, c->Kpart);
+        k_syrk<double><<<gk, 256, 0, s>>>((const double*)w.X + off[d], ldx, nbf, naux, kpad, w.Kpart);
       else if (prec == NOPT_PREC_FP32)
-        k_syrk<float><<<gk, 256, 0, s>>>((const float*)c->X + off[d], ldx, nbf, naux, kpad, c->Kpart);
+        k_syrk<float><<<gk, 256, 0, s>>>((const float*)w.X + off[d], ldx, nbf, naux, kpad, w.Kpart);
       else
-        HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)c->X + off[d], ldx,
-                                      nbf, naux, kpad, c->Kpart));
-      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, c->Kpart, c->nsk, c->ntiles, KT, nbf, nullptr, it[d].K);
+        HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)w.X + off[d], ldx,
+                                      nbf, naux, kpad, w.Kpart));
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, c->nsk, c->ntiles, KT, nbf, nullptr, it[d].K);
     }
     HIPCHK(hipGetLastError());
   }
@@ -1215,6 +1333,7 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_half, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_done, hipEventDisableTiming);
   if (e != hipSuccess) {
     delete c;
     return set_error(NOPT_EHIP, std::string("nopt_jk_create: ") + hip_msg(e));
@@ -1271,7 +1390,12 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
 
 void nopt_jk_destroy(nopt_jk_ctx* c) {
   if (!c) return;
+  if (c->pins > 0) {  // a graph still refers to the buffers: freed with the last such graph
+    c->zombie = true;
+    return;
+  }
   (void)hipSetDevice(c->device);
+  if (c->done_set) (void)hipEventSynchronize(c->ev_done);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dev_free(c->B64);
   dev_free(c->X);
@@ -1312,6 +1436,7 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_half) (void)hipEventDestroy(c->ev_half);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->ev_done) (void)hipEventDestroy(c->ev_done);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1333,6 +1458,8 @@ int nopt_jk_set_tuning(nopt_jk_ctx* c, const int* params) {
   int rc = check_ctx(c, false);
   if (rc) return rc;
   if (!params) return set_error(NOPT_EINVAL, "nopt_jk_set_tuning: null argument");
+  if (c->pins > 0)
+    return set_error(NOPT_ESTATE, "nopt_jk_set_tuning: a captured HIP graph refers to this context's buffers");
   const int na = (int)c->naux;
   if (params[0] < 1 || params[0] > na || params[1] < 0 || params[1] > na || params[2] < 8 || params[2] % 8 ||
       params[2] > std::max(8, na) || params[3] < 1 || params[3] > na || params[4] < 1 || params[4] > na)
@@ -1352,7 +1479,11 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
   if (rc) return rc;
   if (!dD || !dC || reps < 1) return set_error(NOPT_EINVAL, "nopt_jk_autotune: null input or reps < 1");
   if (t_capturing) return set_error(NOPT_ESTATE, "nopt_jk_autotune: not during graph capture");
+  if (c->pins > 0)
+    return set_error(NOPT_ESTATE, "nopt_jk_autotune: a captured HIP graph refers to this context's buffers");
   hipStream_t s = (hipStream_t)stream;
+  rc = order_in(c, s);
+  if (rc) return rc;
   const bool timing = c->timing;
   c->timing = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1420,6 +1551,7 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = timing;
+  if (!rc) rc = order_out(c, s);
   if (!rc && best_ms) *best_ms = best;
   return rc;
 }
@@ -1464,6 +1596,7 @@ struct nopt_graph {
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<nopt_jk_ctx*> ctxs;  // contexts whose buffers the graph refers to (pinned)
 };
 
 int nopt_graph_capture_begin(int device, void* stream) {
@@ -1473,6 +1606,7 @@ int nopt_graph_capture_begin(int device, void* stream) {
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
   t_capturing = true;
+  t_captured.clear();
   return NOPT_OK;
 }
 
@@ -1482,6 +1616,8 @@ int nopt_graph_capture_end(int device, void* stream, nopt_graph** out) {
   *out = nullptr;
   if (!t_capturing) return set_error(NOPT_ESTATE, "nopt_graph_capture_end: no capture in progress");
   t_capturing = false;
+  std::vector<nopt_jk_ctx*> ctxs;
+  ctxs.swap(t_captured);
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamEndCapture((hipStream_t)stream, &g));
   hipGraphExec_t ex = nullptr;
@@ -1499,6 +1635,8 @@ int nopt_graph_capture_end(int device, void* stream, nopt_graph** out) {
   gr->device = device;
   gr->graph = g;
   gr->exec = ex;
+  gr->ctxs.swap(ctxs);
+  for (nopt_jk_ctx* c : gr->ctxs) ++c->pins;
   *out = gr;
   return NOPT_OK;
 }
@@ -1507,15 +1645,29 @@ int nopt_graph_launch(nopt_graph* g, void* stream) {
   nopt::clear_error();
   if (!g) return set_error(NOPT_EINVAL, "nopt_graph_launch: null graph");
   HIPCHK(hipSetDevice(g->device));
-  HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  hipStream_t s = (hipStream_t)stream;
+  for (nopt_jk_ctx* c : g->ctxs) {
+    int rc = order_in(c, s);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGraphLaunch(g->exec, s));
+  for (nopt_jk_ctx* c : g->ctxs) {
+    int rc = order_out(c, s);
+    if (rc) return rc;
+  }
   return NOPT_OK;
 }
 
 void nopt_graph_destroy(nopt_graph* g) {
   if (!g) return;
   (void)hipSetDevice(g->device);
+  for (nopt_jk_ctx* c : g->ctxs) {  // the graph may still run: wait before unpinning
+    if (c->done_set) (void)hipEventSynchronize(c->ev_done);
+  }
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (nopt_jk_ctx* c : g->ctxs)
+    if (--c->pins == 0 && c->zombie) nopt_jk_destroy(c);
   delete g;
 }
 
@@ -1555,6 +1707,8 @@ int nopt_jk_set_B(nopt_jk_ctx* c, const double* eri3, const double* metric) {
   int rc = check_ctx(c, false);
   if (rc) return rc;
   if (!eri3 || !metric) return set_error(NOPT_EINVAL, "nopt_jk_set_B: null input");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   const long na = c->naux;
   double* dV = nullptr;
   rc = dev_alloc((void**)&dV, sizeof(double) * na * na, "metric");
@@ -1579,6 +1733,8 @@ int nopt_jk_set_B_fitted(nopt_jk_ctx* c, const double* Bp) {
   int rc = check_ctx(c, false);
   if (rc) return rc;
   if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_set_B_fitted: null input");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   return form_B(c, Bp, nullptr);
 }
 
@@ -1588,6 +1744,8 @@ int nopt_jk_get_B(nopt_jk_ctx* c, double* Bp) {
   if (rc) return rc;
   if (!Bp) return set_error(NOPT_EINVAL, "nopt_jk_get_B: null output");
   if (!c->B64) return set_error(NOPT_ESTATE, "nopt_jk_get_B: the context keeps no FP64 B (NOPT_CTX_INT8_ONLY)");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   double* dA = nullptr;
   const long total = c->naux * c->npair;
   rc = dev_alloc((void**)&dA, sizeof(double) * total, "packed B");
@@ -1609,6 +1767,8 @@ int nopt_jk_active_ints(nopt_jk_ctx* c, const double* C1, const double* C2, long
   if (!C1 || !C2 || !ints) return set_error(NOPT_EINVAL, "nopt_jk_active_ints: null pointer");
   if (na < 1 || na > 32) return set_error(NOPT_EINVAL, "nopt_jk_active_ints: need 1 <= na <= 32");
   if (!c->B64) return set_error(NOPT_ESTATE, "nopt_jk_active_ints: the context keeps no FP64 B (NOPT_CTX_INT8_ONLY)");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   const long nbf = c->nbf, naux = c->naux, nn = na * na;
   double *dC = nullptr, *dR = nullptr, *dI = nullptr;
   rc = dev_alloc((void**)&dC, sizeof(double) * 2 * na * nbf, "active orbitals");
@@ -1662,7 +1822,10 @@ int nopt_jk_build_device(nopt_jk_ctx* c, const double* dD, const double* dC, lon
   int rc = check_ctx(c, true);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
-  return enqueue_build(c, dD, dC, nocc, dJ, dK, prec, s);
+  rc = order_in(c, s);
+  if (!rc) rc = enqueue_build(c, dD, dC, nocc, dJ, dK, prec, s);
+  if (!rc) rc = order_out(c, s);
+  return rc;
 }
 
 int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, const double* const* dC,
@@ -1677,7 +1840,10 @@ int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, 
     it[d] = BuildItem{dD ? dD[d] : nullptr, dC ? dC[d] : nullptr, nocc[d], dJ ? dJ[d] : nullptr,
                       dK ? dK[d] : nullptr};
   hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
-  return enqueue_batch(c, nd, it, prec, s);
+  rc = order_in(c, s);
+  if (!rc) rc = enqueue_batch(c, nd, it, prec, s);
+  if (!rc) rc = order_out(c, s);
+  return rc;
 }
 
 int nopt_jk_build_batch(nopt_jk_ctx* c, int nd, const double* const* D, const double* const* C, const long* nocc,
@@ -1687,6 +1853,8 @@ int nopt_jk_build_batch(nopt_jk_ctx* c, int nd, const double* const* D, const do
   if (rc) return rc;
   if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
   if (!nocc) return set_error(NOPT_EINVAL, "nopt_jk_build_batch: null nocc array");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   const long nn = c->nbf * c->nbf;
   BuildItem it[kMaxBatch];
   for (int d = 0; d < nd; ++d) {
@@ -1752,6 +1920,8 @@ int nopt_jk_fock_energy(nopt_jk_ctx* c, const double* H, const double* D, const 
   int rc = check_ctx(c, false);
   if (rc) return rc;
   if (!H || !D) return set_error(NOPT_EINVAL, "nopt_jk_fock_energy: H and D are required");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   const long nn = c->nbf * c->nbf;
   HIPCHK(hipMemcpyAsync(c->dH, H, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->dD, D, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
@@ -1798,6 +1968,8 @@ int nopt_jk_energy_gradient(nopt_jk_ctx* c, const double* H, const double* C, lo
   if (rc) return rc;
   if (!H || !C || nocc <= 0 || nocc > c->nbf)
     return set_error(NOPT_EINVAL, "nopt_jk_energy_gradient: H, C and 0 < nocc <= nbf are required");
+  rc = order_in(c, c->stream);
+  if (rc) return rc;
   const long nbf = c->nbf, nn = nbf * nbf;
   rc = upload(c, &c->dC, &c->dC_cap, C, nocc * nbf);
   if (rc) return rc;
@@ -1824,7 +1996,11 @@ int nopt_jk_energy_gradient_device(nopt_jk_ctx* c, const double* dH, const doubl
   if (rc) return rc;
   if (!dH || !dC || !dE || nocc <= 0 || nocc > c->nbf)
     return set_error(NOPT_EINVAL, "nopt_jk_energy_gradient_device: H, C, E and 0 < nocc <= nbf are required");
-  return enqueue_energy_gradient(c, dH, dC, nocc, prec, dE, dG, dF, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  rc = order_in(c, s);
+  if (!rc) rc = enqueue_energy_gradient(c, dH, dC, nocc, prec, dE, dG, dF, s);
+  if (!rc) rc = order_out(c, s);
+  return rc;
 }
 
 }  // extern "C"

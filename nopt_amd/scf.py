@@ -55,6 +55,7 @@ def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, 
             Co = C[:, :nocc]
             D = Co @ Co.T
             J, K = ctx.build(D, Co.T, nocc, prec=cur)
+            built = cur  # precision of the build that gave this iteration's energy
             builds[cur] = builds.get(cur, 0) + 1
             F, e_el = ctx.fock_energy(H, D, J, K, 2.0, 1.0)
             E = e_el + mol.e_nuc
@@ -91,8 +92,10 @@ def rhf(mol, ctx=None, prec="fp64", switch_err=1e-5, e_conv=1e-10, d_conv=1e-8, 
             Fd = sum(c * f for c, f in zip(cf, fs))
             eps, Cp = np.linalg.eigh(X.T @ Fd @ X)
             C = X @ Cp
+        # converged only on an FP64 iteration: a run that switches to the FP64
+        # end-game at its last iteration has not refined its energy
         return SCFResult(E=E, C=C, eps=eps, D=D, F=F, H=H, S=S, iters=it, builds=builds, history=hist,
-                         converged=done, switch_iter=switch_iter)
+                         converged=bool(done and built == "fp64"), switch_iter=switch_iter)
     finally:
         if own:
             ctx.close()

@@ -4,6 +4,7 @@ import ctypes
 import glob
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -41,3 +42,25 @@ def test_error_path_without_gpu_work():
     rc = lib.nopt_mol_create(0, None, None, None, None, 1, None)
     assert rc == L.NOPT_EINVAL
     assert b"null" in lib.nopt_last_error()
+
+
+# C++ (mangled) names of NOPT's include/RI.h:10-16, exported beside the C ones
+RI_H_CXX = ["_Z15RI_core_reallocll", "_Z20DM_to_F_transform_RIPdS_S_S_ll",
+            "_Z19calc_2el_MO_INTS_RIlPdS_S_S_S_S_S_lll"]
+
+
+@pytest.mark.parametrize("name", RI_H_CXX)
+def test_library_exports_ri_h_cxx_names(name):
+    lib = ctypes.CDLL(L.LIB_PATH)
+    assert hasattr(lib, name)
+
+
+def test_cxx_caller_of_ri_h_links_unchanged(tmp_path):
+    """A C++ translation unit that declares the three functions exactly as
+    NOPT's RI.h does compiles, links against libnoptjk.so and runs (no RI
+    context: each call returns an error before any device work)."""
+    exe = str(tmp_path / "ri_h_caller")
+    libdir = os.path.dirname(L.LIB_PATH)
+    subprocess.check_call(["g++", "-std=c++17", os.path.join(ROOT, "tests", "cxx", "ri_h_caller.cpp"), "-o", exe,
+                           "-L" + libdir, "-l:libnoptjk.so", "-Wl,-rpath," + libdir])
+    assert subprocess.run([exe], timeout=60).returncode == 0

@@ -84,6 +84,22 @@ def test_lanczos_matches_dense():
         assert np.allclose(e, ref, rtol=0, atol=1e-10)
 
 
+def test_solve_rejects_missing_spin_states():
+    """Requests the determinants cannot meet raise at once instead of
+    widening the Lanczos search forever: a multiplicity no state has, and more
+    states of a multiplicity than exist (6 electrons in 7 orbitals have seven
+    septets)."""
+    h, g = random_integrals(7, 6, 11)
+    sp = ActiveSpace(7, 3, 3)
+    for mult in (2, 9):
+        with pytest.raises(ValueError):
+            sp.solve(h, g, 1, mult)
+    small = ActiveSpace(4, 2, 2)  # 36 determinants: dense path
+    hs, gs = random_integrals(4, 4, 12)
+    with pytest.raises(ValueError):
+        small.solve(hs, gs, 2, 5)  # a single quintet exists
+
+
 class _NumpyJK:
     """RI-J/K over the host fit B = L^-1 (P|mn) (numpy, for the host test):
     J(D) = sum_Q B_Q (B_Q . D), K = sum_Q (B_Q C^T)(B_Q C^T)^T."""

@@ -1,0 +1,178 @@
+"""The five benchmark workloads, each checked end to end on the GPU.
+
+1. H2O / cc-pVDZ / cc-pVDZ-JKFIT RI-RHF from the core guess: the converged
+   GPU energy equals the oracle's converged RI-RHF energy (NOPT's own path:
+   V^-1/2 per build, src/RI.cpp:115-176, in the D5 subspace) to 1e-10 Eh.
+2. C20H42 at full size in NOPT's Cartesian representation (n_ao 510, naux
+   2366, 81 occupied): GPU FP64 and integer builds on a converged density
+   against the oracle's DM_to_F_transform_RI over the oracle's own integrals.
+4. The ragged batch of 32 water clusters with two CASSCF-style densities each,
+   captured in one HIP graph: replays equal the eager builds bitwise, the
+   integer builds stay within 1 uEh of FP64 for all 64 densities, and the
+   clusters of 1-3 waters, built in Cartesian contexts in the same graph,
+   match the oracle.
+5. (H2O)64 (nbf 1536, naux 7424, 320 occupied): the mixed RI-RHF (integer
+   builds, then the FP64 end-game) lands within 1e-8 Eh of the FP64 RI-RHF,
+   and the integer build on the converged density stays within 1 uEh of FP64.
+(The third workload, the energy adjoint, is tests/test_gpu_large.py and
+tests/test_gpu_scf.py.)
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from nopt_amd import geometry as G
+from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
+from nopt_amd.molecule import Molecule
+from nopt_amd.scf import rhf
+from nopt_amd.workloads import ragged_clusters, two_electron_energy
+
+pytestmark = pytest.mark.gpu
+BASIS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nopt_amd", "basis")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_molecule(sym, xyz_ang, aux="cc-pvdz-jkfit"):
+    return O.OracleMolecule.from_angstrom(sym, xyz_ang, os.path.join(BASIS, "cc-pvdz.gbs"),
+                                          os.path.join(BASIS, aux + ".gbs"))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "int8"])
+def test_config1_h2o_rhf_energy_matches_oracle(prec):
+    """Convergence as the workload states it (1e-10 Eh in the energy, 1e-8
+    RMS in the density, core-Hamiltonian guess); the integer run switches to
+    FP64 builds once the DIIS error is below 1e-5."""
+    sym, xyz = G.water()
+    ref = O.rhf(oracle_molecule(sym, xyz), e_conv=1e-12, d_conv=1e-10)
+    m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit")
+    assert m.nbf == 24 and m.naux == 116 and m.nocc == 5
+    res = rhf(m, prec=prec, e_conv=1e-10, d_conv=1e-8)
+    say("H2O", prec, "GPU", res.E, "oracle", ref["E"], "diff", res.E - ref["E"], "builds", res.builds)
+    assert res.converged
+    assert abs(res.E - ref["E"]) < 1e-10
+
+
+@pytest.fixture(scope="module")
+def c20_cartesian():
+    """Converged FP64 RI-RHF of C20H42 in the Cartesian orbital basis (NOPT's
+    representation) through the GPU build, and the oracle's J/K of the same
+    (D, C_occ) from the oracle's integrals and NOPT's V^-1/2 path."""
+    sym, xyz = G.alkane(20)
+    m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit", pure=False)
+    assert (m.nbf, m.naux, m.nocc) == (510, 2366, 81)
+    ctx = JKContext(m.nbf, m.naux)
+    ctx.set_B(m.eri3_packed(), m.metric())
+    res = rhf(m, ctx=ctx, e_conv=1e-9, d_conv=1e-7)
+    say("C20H42 Cartesian FP64 RI-RHF E", res.E, "iterations", res.iters + 1)
+    assert res.converged
+    Co = np.ascontiguousarray(res.C[:, :m.nocc].T)
+    D = Co.T @ Co
+    om = oracle_molecule(sym, xyz)
+    Vm05, _, dropped = O.s05(om.metric())
+    J0, K0 = O.NoptJK(om.eri3_nopt(), Vm05, om.nocc)(D, Co, om.nocc)
+    yield m, ctx, D, Co, J0, K0, om.e_nuc
+    ctx.close()
+
+
+# FP64 tolerances: NOPT's V^-1/2 form (an eigendecomposition of the JKFIT
+# metric, condition number 2.4e7 here) against the Cholesky form of the same
+# fit differs on the CPU by max|dJ| 1.3e-11, max|dK| 1.1e-12 and 1.5e-10 Eh
+# in sum D(2J - K) (= 2252 Eh) at a core-guess density of this system.
+@pytest.mark.parametrize("prec,tol_j,tol_k,tol_e", [("fp64", 1e-9, 1e-9, 1e-8), ("int8", 1e-8, 2e-7, 1e-6)])
+def test_config2_c20h42_against_oracle(c20_cartesian, prec, tol_j, tol_k, tol_e):
+    m, ctx, D, Co, J0, K0, e_nuc = c20_cartesian
+    J, K = ctx.build(D, Co, m.nocc, prec=prec)
+    dj, dk = float(np.abs(J - J0).max()), float(np.abs(K - K0).max())
+    dE = two_electron_energy(D, J, K) - two_electron_energy(D, J0, K0)
+    say("C20H42", prec, "vs oracle: max|dJ|", dj, "max|dK|", dk, "dE", dE)
+    assert dj < tol_j and dk < tol_k and abs(dE) < tol_e
+
+
+def test_config4_ragged_batch_in_one_graph():
+    """32 clusters (seed 0), every J/K of both densities in one captured
+    graph together with Cartesian copies of the clusters of at most 3 waters;
+    three replays with eager work between them."""
+    # [waters, molecule, host densities, Cartesian copy's cluster index or -1,
+    #  ctx, device inputs, device outputs]
+    systems = []
+    for n, m, dens in ragged_clusters(32, 16, 0):
+        systems.append([n, m, dens, -1])
+    for k, (n, m, dens) in enumerate(ragged_clusters(32, 16, 0, pure=False)):
+        if n <= 3:
+            systems.append([n, m, dens, k])
+    for s in systems:
+        n, m, dens, cart = s
+        ctx = JKContext(m.nbf, m.naux)
+        ctx.set_B(m.eri3_packed(), m.metric())
+        dev = [DeviceArray.from_host(a) if a.size else DeviceArray((1,)) for a in dens]
+        out = [DeviceArray((m.nbf, m.nbf)) for _ in range(4)]
+        s += [ctx, dev, out]
+    say("ragged batch:", len(systems), "systems,", sum(1 for s in systems if s[3] >= 0), "Cartesian")
+    stream = systems[0][4].stream()
+
+    def enqueue(prec):
+        for n, m, dens, cart, ctx, dev, out in systems:
+            dDc, dCc, dDt, dCt = dev
+            Jc, Kc, Jt, Kt = out
+            ctx.build_batch_device([(dDc.ptr, dCc.ptr, dens[1].shape[0], Jc.ptr, Kc.ptr),
+                                    (dDt.ptr, dCt.ptr, dens[3].shape[0], Jt.ptr, Kt.ptr)], prec, stream)
+
+    def outputs():
+        return [[o.to_host() for o in s[6]] for s in systems]
+
+    results = {}
+    for prec in ("fp64", "int8"):
+        enqueue(prec)
+        synchronize()
+        eager = outputs()
+        with Graph.capture(stream) as cap:
+            enqueue(prec)
+        g = cap.graph
+        for rep in range(3):
+            for s in systems:
+                for o in s[6]:
+                    o.copy_from(np.zeros(o.shape))
+            g.launch(stream)
+            synchronize()
+            worst = max(float(np.abs(a - b).max()) for r, e in zip(outputs(), eager) for a, b in zip(r, e))
+            say(prec, "replay", rep, "max|replay - eager|", worst)
+            assert worst == 0.0
+            systems[0][4].build(systems[0][2][0], systems[0][2][1], systems[0][2][1].shape[0], prec="fp64")
+        g.close()
+        results[prec] = eager
+    # integer against FP64: the two-electron energy of every density
+    worst_dE, worst_dK = 0.0, 0.0
+    for s, r64, r8 in zip(systems, results["fp64"], results["int8"]):
+        Dc, Cc, Dt, Ct = s[2]
+        for total, D, J64, K64, J8, K8 in ((False, Dc, r64[0], r64[1], r8[0], r8[1]),
+                                           (True, Dt, r64[2], r64[3], r8[2], r8[3])):
+            de = abs(two_electron_energy(D, J8, K8, total) - two_electron_energy(D, J64, K64, total))
+            worst_dE = max(worst_dE, de)
+            worst_dK = max(worst_dK, float(np.abs(K8 - K64).max()))
+    say("int8 vs fp64 over", 2 * len(systems), "densities: max|dE|", worst_dE, "max|dK|", worst_dK)
+    assert worst_dE < 1e-6 and worst_dK < 5e-7
+    # the Cartesian small clusters against the oracle (NOPT's path, the
+    # oracle's integrals), FP64 replay outputs
+    for s, r64 in zip(systems, results["fp64"]):
+        n, m, (Dc, Cc, Dt, Ct), k = s[:4]
+        if k < 0:
+            continue
+        om = oracle_molecule(*G.water_cluster(n, seed=1 + k))  # the placement of ragged_clusters
+        assert om.n_ao == m.nbf
+        jk = O.NoptJK(om.eri3_nopt(), O.s05(om.metric())[0], max(Ct.shape[0], 1))
+        for D, C, J, K in ((Dc, Cc, r64[0], r64[1]), (Dt, Ct, r64[2], r64[3])):
+            J0, K0 = jk(D, C if C.shape[0] else np.zeros((1, m.nbf)), C.shape[0])
+            dj, dk = float(np.abs(J - J0).max()), float(np.abs(K - K0).max())
+            say("(H2O)%d Cartesian vs oracle: max|dJ| %.2e max|dK| %.2e" % (n, dj, dk))
+            assert dj < 1e-10 and dk < 1e-10
+    for s in systems:
+        for a in s[5] + s[6]:
+            a.free()
+        s[4].close()

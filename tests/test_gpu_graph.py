@@ -43,6 +43,9 @@ def test_graph_of_two_systems_matches_eager(prec):
         m, ctx, D, Co, b = sys_
         ctx.build_device(b["D"].ptr, b["C"].ptr, m.nocc, b["J"].ptr, b["K"].ptr, prec, stream=s)
 
+    # the FP64 build run between replays below, first: a captured context's
+    # buffers cannot grow while the graph exists
+    sysB[1].build(sysB[2], sysB[3], sysB[0].nocc, prec="fp64")
     for x in systems:  # eager builds (also size every buffer before capture)
         enqueue(x)
     synchronize()

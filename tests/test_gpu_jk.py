@@ -171,18 +171,64 @@ def test_nopt_ri_entry_points_match_oracle(c2h4_case):
                                     om.nocc, n) < 0
 
 
-@pytest.mark.parametrize("t", [0, 1])
-def test_nopt_mo_ints_through_the_shim(c2h4_case, t):
-    """The MO_INTS half of calc_2el_MO_INTS_RI through include/nopt_ri.h
-    (orbitals as rows, t = 0, or as columns, t = 1; two different random
-    active sets of 6) against the oracle's restatement of src/RI.cpp:197-226,
-    and nopt_jk_active_ints through JKContext against the same; J and K stay
-    untouched when they are NULL."""
+def test_ri_h_cxx_names_build_like_the_c_entry_points(c2h4_case):
+    """The C++-linkage functions with NOPT's include/RI.h:10-16 signatures
+    (what NOPT's callers link to) give J, K and MO_INTS bitwise equal to the
+    C entry points of nopt_ri.h, and J/K equal to the oracle's restatement."""
+    import ctypes
     from nopt_amd import _lib as L
     om, r, pm = c2h4_case
     lib = L.lib()
-    n, na = pm.nbf, 6
-    rng = np.random.default_rng(5)
+    dp, lg = L._c_double_p, ctypes.c_long
+    f_real = getattr(lib, "_Z15RI_core_reallocll")
+    f_real.restype, f_real.argtypes = ctypes.c_int, [lg, lg]
+    f_dm = getattr(lib, "_Z20DM_to_F_transform_RIPdS_S_S_ll")
+    f_dm.restype, f_dm.argtypes = ctypes.c_int, [dp, dp, dp, dp, lg, lg]
+    f_mo = getattr(lib, "_Z19calc_2el_MO_INTS_RIlPdS_S_S_S_S_S_lll")
+    f_mo.restype, f_mo.argtypes = ctypes.c_int, [lg] + [dp] * 7 + [lg, lg, lg]
+    D, Cc = r["D"], r["C"]
+    MO = np.ascontiguousarray(Cc.T)
+    D = np.ascontiguousarray(D)
+    J0, K0 = r["jk"](D, MO, om.nocc)
+    n, na = pm.nbf, 5
+    V1 = np.ascontiguousarray(np.random.default_rng(9).standard_normal((na, n)))
+    V, A = pm.metric(), np.ascontiguousarray(pm.eri3_nopt())
+    assert lib.RI_gpu_attach(L.dptr(V), L.dptr(A), n, pm.naux, 0) == 0
+    try:
+        assert f_real(om.nocc, n) == 0
+        Jc, Kc = np.zeros((n, n)), np.zeros((n, n))
+        Jx, Kx = np.zeros((n, n)), np.zeros((n, n))
+        assert lib.DM_to_F_transform_RI(L.dptr(Jc), L.dptr(Kc), L.dptr(D), L.dptr(MO), om.nocc, n) == 1
+        assert f_dm(L.dptr(Jx), L.dptr(Kx), L.dptr(D), L.dptr(MO), om.nocc, n) == 1
+        Ic, Ix = np.zeros((na,) * 4), np.zeros((na,) * 4)
+        J2, K2 = np.zeros((n, n)), np.zeros((n, n))
+        assert lib.calc_2el_MO_INTS_RI(n, None, None, None, L.dptr(Ic), None, L.dptr(V1), L.dptr(V1), 0, na, 0) == 0
+        assert f_mo(n, L.dptr(D), L.dptr(J2), L.dptr(K2), L.dptr(Ix), L.dptr(MO), L.dptr(V1), L.dptr(V1),
+                    om.nocc, na, 0) == 0
+    finally:
+        lib.regen_RI_AA_release()
+    say("C++ names: max|dJ| vs oracle", np.abs(Jx - J0).max(), "max|dK|", np.abs(Kx - K0).max())
+    assert np.array_equal(Jx, Jc) and np.array_equal(Kx, Kc)
+    assert np.array_equal(J2, Jc) and np.array_equal(K2, Kc)
+    assert np.array_equal(Ix, Ic)
+    assert np.abs(Jx - J0).max() < 1e-10 and np.abs(Kx - K0).max() < 1e-10
+    assert f_real(om.nocc, n) < 0  # released: no context
+
+
+@pytest.mark.parametrize("na", [6, 9, 16, 17, 32])
+@pytest.mark.parametrize("t", [0, 1])
+def test_nopt_mo_ints_through_the_shim(c2h4_case, t, na):
+    """The MO_INTS half of calc_2el_MO_INTS_RI through include/nopt_ri.h
+    (orbitals as rows, t = 0, or as columns, t = 1; two different random
+    active sets of na orbitals, covering each of the kernel's 8-, 16- and
+    32-wide variants and both sides of their limits) against the oracle's
+    restatement of src/RI.cpp:197-226, and nopt_jk_active_ints through
+    JKContext against the same; J and K stay untouched when they are NULL."""
+    from nopt_amd import _lib as L
+    om, r, pm = c2h4_case
+    lib = L.lib()
+    n = pm.nbf
+    rng = np.random.default_rng(5 + na)
     V1, V2 = rng.standard_normal((na, n)), rng.standard_normal((na, n))
     ref = O.nopt_mo_ints(om.eri3_nopt(), O.s05(om.metric())[0], V1, V2)
     V, A = pm.metric(), np.ascontiguousarray(pm.eri3_nopt())
@@ -199,7 +245,7 @@ def test_nopt_mo_ints_through_the_shim(c2h4_case, t):
         ctx.set_B(pm.eri3_packed(), pm.metric())
         out2 = ctx.active_ints(V1, V2)
     scale = np.abs(ref).max()
-    say("MO_INTS t", t, "shim rel", np.abs(out - ref).max() / scale, "JKContext rel", np.abs(out2 - ref).max() / scale)
+    say("MO_INTS na", na, "t", t, "shim rel", np.abs(out - ref).max() / scale, "JKContext rel", np.abs(out2 - ref).max() / scale)
     assert np.abs(out - ref).max() < 1e-10 * scale
     assert np.abs(out2 - ref).max() < 1e-10 * scale
 
