@@ -114,6 +114,22 @@ int nopt_jk_build(nopt_jk_ctx* ctx, const double* D, const double* C, long nocc,
 int nopt_jk_build_device(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc,
                          double* dJ, double* dK, int prec, void* stream);
 
+/* The build of nopt_jk_build_device without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). No counterpart in NOPT. Its work
+ * buffers (staged D and C, X and its digit planes, the gamma, J and K
+ * partials) live in one static device array of nopt_jk_static_scratch_bytes()
+ * bytes (256 MiB) per GPU, held from the library's first use of that GPU
+ * until the process ends; a build that needs more fails with NOPT_EINVAL
+ * before any device work (e.g. C20H42 in FP64 needs about 0.9 GB: use
+ * nopt_jk_build_device). Every call on a GPU shares that GPU's array, so
+ * such calls that overlap on one GPU (different streams, different contexts)
+ * race; calls on one stream run one after the other. Results equal those of
+ * nopt_jk_build_device. */
+int nopt_jk_build_device_static(nopt_jk_ctx* ctx, const double* dD, const double* dC, long nocc,
+                                double* dJ, double* dK, int prec, void* stream);
+long nopt_jk_static_scratch_bytes(void);
+
 /* Several builds against the same B in one pass (NOPT calls the build twice
  * per CASSCF macro-iteration, for the inactive and the total density,
  * src/CAS.cpp:696-707, and a signed factor of an indefinite density needs a

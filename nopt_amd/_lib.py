@@ -58,6 +58,8 @@ GPU_SYMBOLS = [
     ("nopt_jk_build", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, _c_double_p,
                                      _c_double_p, ctypes.c_int]),
     ("nopt_jk_build_device", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp, _vp, ctypes.c_int, _vp]),
+    ("nopt_jk_build_device_static", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_long, _vp, _vp, ctypes.c_int, _vp]),
+    ("nopt_jk_static_scratch_bytes", ctypes.c_long, []),
     ("nopt_jk_build_batch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_c_double_p),
                                            ctypes.POINTER(_c_double_p), _c_long_p, ctypes.POINTER(_c_double_p),
                                            ctypes.POINTER(_c_double_p), ctypes.c_int]),

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1271,6 +1272,35 @@ int upload(nopt_jk_ctx* c, double** dst, long* cap, const double* src, long coun
 
 }  // namespace
 
+namespace {
+// Work space of the allocation-free build (nopt_jk_build_device_static): one
+// array per GPU, part of the library's code object, held until the process
+// ends. In host code `scratch` only names the symbol; its device address on
+// each GPU comes from hipGetSymbolAddress (scratch_address) and is the only
+// form in which it reaches kernels or copies.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's scratch array (the device must be
+// current): looked up on the first call on each device and kept.
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return set_error(NOPT_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(NOPT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hip_msg(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return NOPT_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int nopt_jk_create(int device, long nbf, long naux, nopt_jk_ctx** out) {
@@ -1827,6 +1857,24 @@ int nopt_jk_build_device(nopt_jk_ctx* c, const double* dD, const double* dC, lon
   if (!rc) rc = order_out(c, s);
   return rc;
 }
+
+int nopt_jk_build_device_static(nopt_jk_ctx* c, const double* dD, const double* dC, long nocc, double* dJ,
+                                double* dK, int prec, void* stream) {
+  nopt::clear_error();
+  int rc = check_ctx(c, true);
+  if (rc) return rc;
+  void* work = nullptr;
+  rc = scratch_address(c->device, &work);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
+  const BuildItem x{dD, dC, nocc, dJ, dK};
+  rc = order_in(c, s);
+  if (!rc) rc = enqueue_batch(c, 1, &x, prec, s, work, kScratchBytes);
+  if (!rc) rc = order_out(c, s);
+  return rc;
+}
+
+long nopt_jk_static_scratch_bytes(void) { return (long)kScratchBytes; }
 
 int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, const double* const* dC,
                                const long* nocc, double* const* dJ, double* const* dK, int prec, void* stream) {
