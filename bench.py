@@ -83,7 +83,7 @@ KERNELS = {
     "coulomb_j": {"fp64": "k_jpart<1>", "fp32": "k_jpart<1>", "bf16x6": "k_jpart<1>", "bf16x3": "k_jpart<1>",
                   "int8": "k_jpart_slots<1, 2, true>"},
     "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
-                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8<8>"},
+                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8<4>"},
 }
 
 
@@ -178,11 +178,12 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
     (all but 5 doubly occupied core-guess orbitals) and a total density with
     an active space of 10 orbitals and a random PSD 1-RDM of trace 10
     (factor [sqrt(2) C_core; NO sqrt(n/2)], src/molecule.cpp:2959-2977). All
-    2*count builds are captured in one HIP graph; builds/s of the replayed
-    graph and of the same builds launched one by one, with the split counts
-    of each system set by the autotuner (the replay with the counts picked at
-    context creation is reported beside it)."""
-    from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
+    2*count builds are captured in one HIP graph from one
+    nopt_jk_build_ragged_device call; builds/s of the replayed graph and of the
+    builds launched one by one, with the split counts of each system set by
+    the autotuner (the replay with the counts picked at context creation is
+    reported beside it)."""
+    from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize, build_ragged_device
     from nopt_amd.workloads import ragged_clusters
     t0 = time.perf_counter()
     systems, sizes = [], []
@@ -204,11 +205,19 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
             d["ctx"].build_batch_device([(dDc.ptr, dCc.ptr, d["ncore"], Jc.ptr, Kc.ptr),
                                          (dDt.ptr, dCt.ptr, d["nt"], Jt.ptr, Kt.ptr)], p, stream)
 
-    def graph_ms(p):
-        enqueue(p)  # sizes every buffer before the capture
+    def ragged(p):
+        # every system in one call
+        build_ragged_device([(d["ctx"], [(d["dev"][0].ptr, d["dev"][1].ptr, d["ncore"], d["out"][0].ptr,
+                                          d["out"][1].ptr),
+                                         (d["dev"][2].ptr, d["dev"][3].ptr, d["nt"], d["out"][2].ptr,
+                                          d["out"][3].ptr)]) for d in systems], p, stream)
+
+    def graph_ms(p, fn=None):
+        fn = fn or ragged
+        fn(p)  # sizes every buffer before the capture
         synchronize()
         with Graph.capture(stream) as cap:
-            enqueue(p)
+            fn(p)
         g = cap.graph
         g.launch(stream)
         synchronize()
@@ -240,7 +249,7 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
         eager = (time.perf_counter() - t1) / steps
         ref = [[o.to_host() for o in d["out"]] for d in systems]
         with Graph.capture(stream) as cap:
-            enqueue(p)
+            ragged(p)
         g = cap.graph
         g.launch(stream)
         synchronize()

@@ -148,6 +148,18 @@ int nopt_jk_build_batch_device(nopt_jk_ctx* ctx, int nd, const double* const* dD
                                const double* const* dC, const long* nocc, double* const* dJ,
                                double* const* dK, int prec, void* stream);
 
+/* A ragged batch of systems in one call (no counterpart in NOPT, which builds
+ * one system at a time): system k builds the nd[k] densities of its own
+ * context ctxs[k], items as in nopt_jk_build_batch_device; the item arrays
+ * (dD, dC, nocc, dJ, dK) are flattened over the systems in order, system k's
+ * items following those of systems 0..k-1. Every argument is checked before
+ * anything is enqueued; the systems' builds are then enqueued on `stream` one
+ * after the other, so a capture on `stream` records the whole batch as one
+ * HIP graph. All contexts are on one device. */
+int nopt_jk_build_ragged_device(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const double* const* dD,
+                                const double* const* dC, const long* nocc, double* const* dJ,
+                                double* const* dK, int prec, void* stream);
+
 /* Fock assembly and energy on the device from host buffers:
  * F = H + cj*J - ck*K and E = sum D.*F + sum D.*H (E_1el_calc,
  * src/doCI_matr.cpp:38-47, as used at src/SCF.cpp:134; the caller adds

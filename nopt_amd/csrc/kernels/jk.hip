@@ -1032,10 +1032,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   // integer mode with one column group: the half-transform writes the digit
   // planes of X directly (per density), X itself is never stored; pairs of
   // auxiliary functions share the exponent of a row m and lie side by side
-  // (npq rows of pairs), so the exchange streams 2*nocc orbitals per exponent
+  // (npq rows of pairs), so the exchange streams 2*nocc orbitals per exponent.
+  // (Triples of functions pad less for 81 orbitals, 256 columns per 3 against
+  // 192 per 2, but their half-transform, one block of 12 waves per CU, was
+  // slower than the exchange gained: 535 against 557 builds/s at C20H42.)
   const bool xfused = i8 && anyK && ngroups == 1;
-  const long npq = (naux + 1) / 2, rowsq = npq * nbf;
-  auto ldk_of = [&](long nocc) { return xfused ? round_up(2 * nocc, 64) : round_up(nocc, 64); };
+  const int qpb = xfused ? 2 : 1;
+  const long npq = (naux + qpb - 1) / qpb, rowsq = npq * nbf;
+  auto ldk_of = [&](long nocc) { return xfused ? round_up(qpb * nocc, 64) : round_up(nocc, 64); };
   long xoff[kMaxBatch + 1];  // digit-plane offsets (bytes) of each density
   xoff[0] = 0;
   for (int d = 0; d < nd; ++d)
@@ -1183,7 +1187,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
           xs.xe[k] = (int*)w.Xe + (long)d * rowsq;
         }
       if (xfused) gh.y = (unsigned)npq;
-      e = nopt::launch_half_i8(nt, xfused ? 2 : 1, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux,
+      e = nopt::launch_half_i8(nt, qpb, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux,
                                (const int8_t*)w.Cd, ldx * c->ld8, (const int*)w.Ce,
                                xfused ? nullptr : (double*)w.X, ldx, xs);
     }
@@ -1223,7 +1227,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       if (!xfused)
         HIPCHK(nopt::split_rows_i8((const double*)w.X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
                                    rows, rows * ldk, (int*)xe, s));
-      // the exchange runs over the rows of the digit planes: pairs of
+      // the exchange runs over the rows of the digit planes: groups of qpb
       // functions (npq of them) when fused, single functions otherwise
       const long nq = xfused ? npq : naux;
       HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, nq * nbf * ldk, xe, ldk, nbf, (int)nq, w.Kpart));
@@ -1892,6 +1896,40 @@ int nopt_jk_build_batch_device(nopt_jk_ctx* c, int nd, const double* const* dD, 
   if (!rc) rc = enqueue_batch(c, nd, it, prec, s);
   if (!rc) rc = order_out(c, s);
   return rc;
+}
+
+int nopt_jk_build_ragged_device(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const double* const* dD,
+                                const double* const* dC, const long* nocc, double* const* dJ, double* const* dK,
+                                int prec, void* stream) {
+  nopt::clear_error();
+  if (nsys < 1 || !ctxs || !nd || !nocc) return set_error(NOPT_EINVAL, "nopt_jk_build_ragged_device: empty batch");
+  const int dev = ctxs[0] ? ctxs[0]->device : 0;
+  for (int k = 0; k < nsys; ++k) {  // every argument checked before any work is enqueued
+    int rc = check_ctx(ctxs[k], true);
+    if (rc) return rc;
+    if (ctxs[k]->device != dev)
+      return set_error(NOPT_EINVAL, "nopt_jk_build_ragged_device: all contexts must be on one device");
+    if (nd[k] < 1 || nd[k] > kMaxBatch)
+      return set_error(NOPT_EINVAL, "nopt_jk_build_ragged_device: 1 <= nd[k] <= NOPT_MAX_BATCH");
+  }
+  HIPCHK(hipSetDevice(dev));
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default (null) stream
+  long first = 0;
+  for (int k = 0; k < nsys; ++k) {
+    nopt_jk_ctx* c = ctxs[k];
+    BuildItem it[kMaxBatch];
+    for (int d = 0; d < nd[k]; ++d) {
+      const long i = first + d;
+      it[d] = BuildItem{dD ? dD[i] : nullptr, dC ? dC[i] : nullptr, nocc[i], dJ ? dJ[i] : nullptr,
+                        dK ? dK[i] : nullptr};
+    }
+    first += nd[k];
+    int rc = order_in(c, s);
+    if (!rc) rc = enqueue_batch(c, nd[k], it, prec, s);
+    if (!rc) rc = order_out(c, s);
+    if (rc) return rc;
+  }
+  return NOPT_OK;
 }
 
 int nopt_jk_build_batch(nopt_jk_ctx* c, int nd, const double* const* D, const double* const* C, const long* nocc,

@@ -891,6 +891,9 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
       xb[j] = n < nbf ? e[n] : 0;
     }
   };
+  // a wave whose block lies wholly above the diagonal of a diagonal tile
+  // computes nothing there (k_finalize reads only the lower triangle)
+  const bool upper = bi == bj && (wr + 1) * 16 * IT <= wc * 32;
   v4i_t acc[IT][2][5];
   d4_t k64[IT][2];
 #pragma unroll
@@ -915,70 +918,71 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
     if (qend) eload(q0 + sl / nkq, ea, eb);
     const int8_t* As = sm[cur];
     const int8_t* Bs = sm[cur] + TB;
-    uint4 av[IT][4];
-#pragma unroll
-    for (int i = 0; i < IT; ++i) {
-      const int r = wr * 16 * IT + i * 16 + lr;
-#pragma unroll
-      for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * KT8 + r) * KS + 16 * swz(r, lc));
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int cidx = wc * 32 + j * 16 + lr;
-      const int8_t* cp = Bs + cidx * KS + 16 * swz(cidx, lc);
-      uint4 bv = *reinterpret_cast<const uint4*>(cp);
+    if (!upper) {
+      uint4 av[IT][4];
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[0] = mfma_i8(av[i][0], bv, A[0]);
-        A[1] = mfma_i8(av[i][1], bv, A[1]);
-        A[2] = mfma_i8(av[i][2], bv, A[2]);
-        A[3] = mfma_i8(av[i][3], bv, A[3]);
+        const int r = wr * 16 * IT + i * 16 + lr;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * KT8 + r) * KS + 16 * swz(r, lc));
       }
-      bv = *reinterpret_cast<const uint4*>(cp + KT8 * KS);
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[1] = mfma_i8(av[i][0], bv, A[1]);
-        A[2] = mfma_i8(av[i][1], bv, A[2]);
-        A[3] = mfma_i8(av[i][2], bv, A[3]);
-        A[4] = mfma_i8(av[i][3], bv, A[4]);
-      }
-      bv = *reinterpret_cast<const uint4*>(cp + 2 * KT8 * KS);
+      for (int j = 0; j < 2; ++j) {
+        const int cidx = wc * 32 + j * 16 + lr;
+        const int8_t* cp = Bs + cidx * KS + 16 * swz(cidx, lc);
+        uint4 bv = *reinterpret_cast<const uint4*>(cp);
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[2] = mfma_i8(av[i][0], bv, A[2]);
-        A[3] = mfma_i8(av[i][1], bv, A[3]);
-        A[4] = mfma_i8(av[i][2], bv, A[4]);
-      }
-      bv = *reinterpret_cast<const uint4*>(cp + 3 * KT8 * KS);
-#pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[3] = mfma_i8(av[i][0], bv, A[3]);
-        A[4] = mfma_i8(av[i][1], bv, A[4]);
-      }
-    }
-    if (qend) {
-      // combine the class sums of this auxiliary function: with |digit| <= 64
-      // and ldk <= 1024, hi = 128 A0 + A1 and mid = 128 A2 + A3 are exact in
-      // int32 and 2^21 hi + 2^7 mid + A4 (= 2^28 x the digit sum) in FP64
-#pragma unroll
-      for (int i = 0; i < IT; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int i = 0; i < IT; ++i) {
           v4i_t* A = acc[i][j];
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int hi = A[0][v] * 128 + A[1][v], mid = A[2][v] * 128 + A[3][v];
-            const double sv = fma((double)hi, 0x1p21, fma((double)mid, 128.0, (double)A[4][v]));
-            k64[i][j][v] += ldexp(sv, ea[i][v] + eb[j] - 40);
-          }
-#pragma unroll
-          for (int c = 0; c < 5; ++c) A[c] = v4i_t{0, 0, 0, 0};
+          A[0] = mfma_i8(av[i][0], bv, A[0]);
+          A[1] = mfma_i8(av[i][1], bv, A[1]);
+          A[2] = mfma_i8(av[i][2], bv, A[2]);
+          A[3] = mfma_i8(av[i][3], bv, A[3]);
         }
-
+        bv = *reinterpret_cast<const uint4*>(cp + KT8 * KS);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          v4i_t* A = acc[i][j];
+          A[1] = mfma_i8(av[i][0], bv, A[1]);
+          A[2] = mfma_i8(av[i][1], bv, A[2]);
+          A[3] = mfma_i8(av[i][2], bv, A[3]);
+          A[4] = mfma_i8(av[i][3], bv, A[4]);
+        }
+        bv = *reinterpret_cast<const uint4*>(cp + 2 * KT8 * KS);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          v4i_t* A = acc[i][j];
+          A[2] = mfma_i8(av[i][0], bv, A[2]);
+          A[3] = mfma_i8(av[i][1], bv, A[3]);
+          A[4] = mfma_i8(av[i][2], bv, A[4]);
+        }
+        bv = *reinterpret_cast<const uint4*>(cp + 3 * KT8 * KS);
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+          v4i_t* A = acc[i][j];
+          A[3] = mfma_i8(av[i][0], bv, A[3]);
+          A[4] = mfma_i8(av[i][1], bv, A[4]);
+        }
+      }
+      if (qend) {
+        // combine the class sums of this auxiliary function: with |digit| <= 64
+        // and ldk <= 1024, hi = 128 A0 + A1 and mid = 128 A2 + A3 are exact in
+        // int32 and 2^21 hi + 2^7 mid + A4 (= 2^28 x the digit sum) in FP64
+#pragma unroll
+        for (int i = 0; i < IT; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            v4i_t* A = acc[i][j];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int hi = A[0][v] * 128 + A[1][v], mid = A[2][v] * 128 + A[3][v];
+              const double sv = fma((double)hi, 0x1p21, fma((double)mid, 128.0, (double)A[4][v]));
+              k64[i][j][v] += ldexp(sv, ea[i][v] + eb[j] - 40);
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) A[c] = v4i_t{0, 0, 0, 0};
+          }
+      }
     }
     if (more) lstore(cur ^ 1);
     __syncthreads();
@@ -1048,7 +1052,7 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_
                           long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
                           long ldx, const XSplit& xs) {
   // ld8 <= 2048 keeps 128 A0 + A1 exact in int32 (the epilogue's class combine)
-  if (ld8 % 64 != 0 || ld8 > 2048 || (long)grid.z * 16 * nt > ldx || (qpb != 1 && qpb != 2))
+  if (ld8 % 64 != 0 || ld8 > 2048 || (long)grid.z * 16 * nt > ldx || qpb < 1 || qpb > 2)
     return hipErrorInvalidValue;
   if ((long)grid.y * qpb < naux || (long)(grid.y - 1) * qpb >= naux) return hipErrorInvalidValue;
   if (xs.nseg < 0 || xs.nseg > 4 || (xs.nseg > 0 && grid.z != 1)) return hipErrorInvalidValue;
@@ -1142,7 +1146,8 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart) {
   if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
-  k_syrk_i8<8><<<(unsigned)(ntiles * nsplit), 512, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
+  // four waves of 32 x 32 (two waves per SIMD, two blocks per CU)
+  k_syrk_i8<4><<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
   return hipGetLastError();
 }
 

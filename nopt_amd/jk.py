@@ -324,6 +324,22 @@ class _Capture:
         return False
 
 
+def build_ragged_device(systems, prec="fp64", stream=0):
+    """One call for a ragged batch (include/nopt_jk.h, nopt_jk_build_ragged_device):
+    `systems` is a list of (JKContext, items) with items as in
+    JKContext.build_batch_device, enqueued on `stream` one system after the
+    other once every argument has been checked."""
+    nsys = len(systems)
+    flat = [it for _, items in systems for it in items]
+    n = len(flat)
+    col = lambda i: (ctypes.c_void_p * n)(*[(it[i] or None) for it in flat])
+    p = L.PREC[prec] if isinstance(prec, str) else int(prec)
+    L.check(L.lib().nopt_jk_build_ragged_device(
+        nsys, (ctypes.c_void_p * nsys)(*[c.handle.value for c, _ in systems]),
+        (ctypes.c_int * nsys)(*[len(items) for _, items in systems]), col(0), col(1),
+        (ctypes.c_long * n)(*[int(it[2]) for it in flat]), col(3), col(4), p, stream or None))
+
+
 def synchronize(device=0):
     L.check(L.lib().nopt_dev_synchronize(device))
 

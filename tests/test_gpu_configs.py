@@ -25,7 +25,7 @@ import pytest
 
 import oracle as O
 from nopt_amd import geometry as G
-from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize
+from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize, build_ragged_device
 from nopt_amd.molecule import Molecule
 from nopt_amd.scf import rhf
 from nopt_amd.workloads import ragged_clusters, two_electron_energy
@@ -97,8 +97,10 @@ def test_config2_c20h42_against_oracle(c20_cartesian, prec, tol_j, tol_k, tol_e)
 
 def test_config4_ragged_batch_in_one_graph():
     """32 clusters (seed 0), every J/K of both densities in one captured
-    graph together with Cartesian copies of the clusters of at most 3 waters;
-    three replays with eager work between them."""
+    graph together with Cartesian copies of the clusters of at most 3 waters
+    (one nopt_jk_build_ragged_device call);
+    three replays with eager work between them, each equal to the builds
+    launched system by system."""
     # [waters, molecule, host densities, Cartesian copy's cluster index or -1,
     #  ctx, device inputs, device outputs]
     systems = []
@@ -124,6 +126,12 @@ def test_config4_ragged_batch_in_one_graph():
             ctx.build_batch_device([(dDc.ptr, dCc.ptr, dens[1].shape[0], Jc.ptr, Kc.ptr),
                                     (dDt.ptr, dCt.ptr, dens[3].shape[0], Jt.ptr, Kt.ptr)], prec, stream)
 
+    def ragged(prec):
+        """every system in one call"""
+        build_ragged_device([(ctx, [(dev[0].ptr, dev[1].ptr, dens[1].shape[0], out[0].ptr, out[1].ptr),
+                                    (dev[2].ptr, dev[3].ptr, dens[3].shape[0], out[2].ptr, out[3].ptr)])
+                             for n, m, dens, cart, ctx, dev, out in systems], prec, stream)
+
     def outputs():
         return [[o.to_host() for o in s[6]] for s in systems]
 
@@ -132,8 +140,16 @@ def test_config4_ragged_batch_in_one_graph():
         enqueue(prec)
         synchronize()
         eager = outputs()
+        for s in systems:
+            for o in s[6]:
+                o.copy_from(np.zeros(o.shape))
+        ragged(prec)  # the one-call form, eagerly
+        synchronize()
+        worst = max(float(np.abs(a - b).max()) for r, e in zip(outputs(), eager) for a, b in zip(r, e))
+        say(prec, "ragged call max|ragged - eager|", worst)
+        assert worst == 0.0
         with Graph.capture(stream) as cap:
-            enqueue(prec)
+            ragged(prec)
         g = cap.graph
         for rep in range(3):
             for s in systems:
