@@ -36,6 +36,7 @@ B_FORM = {"int8": "int8 digit planes (4 planes of the full rows for the half-tra
                   "lower triangle for gamma and J)",
           "fp32": "FP64 rows (rounded to FP32 on the fly)", "bf16x6": "FP64 rows (split to bf16 on the fly)",
           "bf16x3": "FP64 rows (split to bf16 on the fly)", "fp64": "FP64 rows"}
+B_FORM_COMPACT = "a 5-byte packed lower triangle (28-bit integer + last digit), read by every kernel"
 PEAK = {"hbm_GBps": 8000.0, "fp64_TF": 78.6, "fp32_TF": 157.3, "bf16_TF": 2500.0, "int8_TOPS": 5000.0}
 
 
@@ -115,7 +116,8 @@ def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
             return mfma(f_half, PEAK["fp64_TF"])
         if prec == "int8":  # 4 B digit planes in, 4 X digit planes out
             r = hbm(4.0 * naux * (npair if half_packed else nbf * nbf) + 4.0 * naux * nbf * nocc)
-            r["b_form"] = "4 int8 digit planes, " + ("packed lower triangle" if half_packed else "full rows")
+            r["b_form"] = ("packed lower triangle (28-bit integers, unpacked to 4 digit planes in LDS)"
+                           if half_packed else "4 int8 digit planes, full rows")
             r["mfma_view"] = dict(algorithmic_flops=f_half, achieved_TOPs=f_half / t / 1e12,
                                   frac_of_int8_peak=f_half / t / 1e12 / PEAK["int8_TOPS"],
                                   emulation_frac=13 * f_half / t / 1e12 / PEAK["int8_TOPS"])
@@ -490,6 +492,8 @@ def main():
                     help="comma-separated modes to time (fp64 is always built as the reference)")
     ap.add_argument("--no-perturbed", action="store_true", help="skip the perturbed-density accuracy check")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged water-cluster batch")
+    ap.add_argument("--compact", action="store_true",
+                    help="integer form of B as the packed triangle alone (the large-system form)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -536,7 +540,7 @@ def main():
     eri3, metric = mol.eri3_packed(), mol.metric()
     t_ints = time.perf_counter() - t0
     log(f"[bench] C{args.alkane}: nbf {mol.nbf} naux {mol.naux} nocc {mol.nocc}; host integrals {t_ints:.1f} s")
-    ctx = JKContext(mol.nbf, mol.naux)
+    ctx = JKContext(mol.nbf, mol.naux, compact=args.compact)
     t0 = time.perf_counter()
     ctx.set_B(eri3, metric)
     t_B = time.perf_counter() - t0
@@ -593,9 +597,10 @@ def main():
     log(f"[bench] host-buffer path ({prec}): {host_rate:.1f} builds/s")
 
     timed = ("half_transform", "gamma", "coulomb_j", "exchange")
-    dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec)), key=lambda k: phases[k])
-    roof = roofline(dom, phases[dom], nbf, naux, nocc, prec)
-    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p) for k in timed} for p in modes}
+    hp = args.compact
+    dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec, hp)), key=lambda k: phases[k])
+    roof = roofline(dom, phases[dom], nbf, naux, nocc, prec, hp)
+    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p, hp) for k in timed} for p in modes}
     # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) for this workload
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic_c20h42.json")
     if args.alkane == 20 and os.path.exists(tfile):
@@ -641,7 +646,8 @@ def main():
         "data": "C20H42 geometry built in-repo; converged FP64 RI-RHF density and orbitals",
         "config": {
             "workload": f"RI-J/K build, all-trans C{args.alkane}H{2 * args.alkane + 2} cc-pVDZ/cc-pVDZ-JKFIT, "
-                        f"nbf {nbf}, naux {naux}, nocc {nocc}, B resident in HBM as {B_FORM[prec]}",
+                        f"nbf {nbf}, naux {naux}, nocc {nocc}, B resident in HBM as "
+                        f"{B_FORM_COMPACT if (args.compact and prec == 'int8') else B_FORM[prec]}",
             "precision": prec,
             "arithmetic": {"int8": "int8 digit products, exact int32 class sums, fp64 combine; J from 35-bit "
                                    "integers in fp64",

@@ -58,15 +58,20 @@ void nopt_jk_destroy(nopt_jk_ctx* ctx);
 
 /* Which forms of B a context keeps. The default keeps both: FP64 rows (read
  * by the FP64, FP32 and bf16 modes) and the integer forms (read by the
- * integer mode: five int8 digit planes of the full rows, 5 bytes per element,
- * and the packed lower triangle for the Coulomb pair, 5 bytes per triangle
- * element). A large system keeps one of them: FP64 only (8 bytes per
- * element, full rows) or the integer forms only (7.5 bytes per element); a build in
- * a mode whose form is not kept fails with NOPT_ESTATE. NOPT has a single
- * FP64 form of RI_AA (src/RI.cpp:16-32); this choice has no counterpart. */
+ * integer mode): the packed lower triangle, 5 bytes per triangle element,
+ * which the Coulomb pair reads, and, unless the system is large or
+ * NOPT_CTX_COMPACT is given, four int8 digit planes of the full rows,
+ * 4 bytes per element, which the half-transform reads (faster than
+ * unpacking the triangle). The integer forms hold 6.5 bytes per element with
+ * the full rows, 2.5 without. A large system keeps one form: FP64 only
+ * (8 bytes per element, full rows) or the integer forms only; a build in a
+ * mode whose form is not kept fails with NOPT_ESTATE. The full rows are kept
+ * when they take at most 24 GiB. NOPT has a single FP64 form of RI_AA
+ * (src/RI.cpp:16-32); this choice has no counterpart. */
 #define NOPT_CTX_DEFAULT 0
 #define NOPT_CTX_FP64_ONLY 1
 #define NOPT_CTX_INT8_ONLY 2
+#define NOPT_CTX_COMPACT 4 /* or'ed with DEFAULT or INT8_ONLY: the triangle only */
 int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** out);
 
 /* Forms the fitted tensor B = L^-1 (P|mn) on the device in FP64, where

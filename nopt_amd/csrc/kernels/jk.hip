@@ -580,6 +580,9 @@ __global__ void k_density(const double* __restrict__ C, int nocc, int nbf, doubl
 // ================================================================ context
 namespace {
 constexpr int kMaxBatch = NOPT_MAX_BATCH;  // densities per batched build
+// largest full-row digit planes of B a context keeps (larger: the packed
+// triangle alone feeds the half-transform)
+constexpr size_t kFullRowsMax = (size_t)24 << 30;
 }  // namespace
 
 struct nopt_jk_ctx {
@@ -618,12 +621,16 @@ struct nopt_jk_ctx {
   long xD_cap[kMaxBatch - 1] = {}, xC_cap[kMaxBatch - 1] = {}, xJ_cap[kMaxBatch - 1] = {},
        xK_cap[kMaxBatch - 1] = {};
   int* info = nullptr;
-  // int8 digit planes of B (5 x naux*nbf x ld8) with per-row exponents, and
-  // the per-build digit planes of the orbital factor (4 x ldx x ld8)
+  // integer forms of B: the row exponents (Be, and as bytes Be8[Q*ld8 + n]),
+  // the packed lower triangle Bs below, and, when bfull, the full rows as
+  // four digit planes (4 x naux*nbf x ld8) for the half-transform; the
+  // per-build digit planes of the orbital factor (4 x ldx x ld8)
   long ld8 = 0;
   int mtiles64 = 0;
+  bool bfull = false;
   int8_t* Bd = nullptr;
   int* Be = nullptr;
+  int8_t* Be8 = nullptr;
   void* Cd = nullptr;
   size_t Cd_bytes = 0;
   void* Ce = nullptr;
@@ -633,10 +640,10 @@ struct nopt_jk_ctx {
   void* Xe = nullptr;
   size_t Xe_bytes = 0;
   int ntiles8 = 0, nsk8 = 0;  // 64x64 exchange tiles of the integer path
-  // integer Coulomb pair: the lower triangle of B as packed rows in
-  // 16-column segments (five digit planes, plane stride splane), the segment
-  // and slot tables, gamma partials per slot block, Q-splits of the gamma
-  // and J kernels
+  // the lower triangle of B as packed rows in 16-column segments (N28 as
+  // int32, then d4 as bytes: 5*splane bytes), read by the Coulomb pair and,
+  // without bfull, by the half-transform; the segment and slot tables, gamma
+  // partials per slot block, Q-splits of the gamma and J kernels
   int8_t* Bs = nullptr;
   long splane = 0;
   int nseg = 0;
@@ -817,11 +824,12 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
   int rc = NOPT_OK;
   c->b_ready = false;
   if (keep64 && !c->B64) rc = dev_alloc((void**)&c->B64, sizeof(double) * rows * c->ld64, "B (FP64)");
-  if (!rc && keep8 && !c->Bd) {
-    rc = dev_alloc((void**)&c->Bd, (size_t)5 * rows * c->ld8, "B digit planes");
+  if (!rc && keep8 && !c->Be) {
+    if (c->bfull) rc = dev_alloc((void**)&c->Bd, (size_t)4 * rows * c->ld8, "B digit planes");
     if (!rc) rc = dev_alloc((void**)&c->Be, sizeof(int) * rows, "B row exponents");
+    if (!rc) rc = dev_alloc((void**)&c->Be8, (size_t)na * c->ld8, "B row exponents (bytes)");
     c->splane = na * 16L * c->nseg;
-    if (!rc) rc = dev_alloc((void**)&c->Bs, (size_t)5 * c->splane, "packed digit rows of B");
+    if (!rc) rc = dev_alloc((void**)&c->Bs, (size_t)5 * c->splane, "packed lower triangle of B");
   }
   if (rc) return rc;
   double* dA = nullptr;
@@ -832,11 +840,11 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
     hipStream_t s = c->stream;
     if (keep64) HIPCHK(hipMemsetAsync(c->B64, 0, sizeof(double) * rows * c->ld64, s));
     if (rmax) {
-      HIPCHK(hipMemsetAsync(c->Bd, 0, (size_t)5 * rows * c->ld8, s));
+      if (c->Bd) HIPCHK(hipMemsetAsync(c->Bd, 0, (size_t)4 * rows * c->ld8, s));
       HIPCHK(hipMemsetAsync(rmax, 0, sizeof(unsigned long long) * rows, s));
     }
-    // pass 1 forms the FP64 rows or the row maxima, pass 2 (digits without
-    // FP64 rows) the digit planes
+    // pass 1 forms the FP64 rows or the row maxima, pass 2 (integer forms
+    // without FP64 rows) the packed triangle and the digit planes
     for (int pass = 0; pass < (rmax ? 2 : 1); ++pass) {
       for (int m0 = 0; m0 < (int)nbf;) {
         int m1 = m0 + 1;
@@ -845,20 +853,26 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
         HIPCHK(hipMemcpy2DAsync(dA, sizeof(double) * w, host + p0, sizeof(double) * c->npair, sizeof(double) * w,
                                 na, hipMemcpyHostToDevice, s));
         if (dL) HIPCHK(nopt::trsm_lower_f64(dL, na, (int)na, dA, w, w, s));
-        if (keep64)
+        if (keep64) {
           HIPCHK(nopt::unpack_packed_f64(dA, w, m0, m1, (int)nbf, na, c->B64, c->ld64, s));
-        else if (pass == 0)
+        } else if (pass == 0) {
           HIPCHK(nopt::rowmax_packed(dA, w, m0, m1, (int)nbf, na, rmax, s));
-        else
-          HIPCHK(nopt::split_packed_i8(dA, w, m0, m1, (int)nbf, na, c->Be, 5, c->Bd, c->ld8, rows * c->ld8, s));
+        } else {
+          HIPCHK(nopt::pack_segs_chunk(dA, w, m0, m1, (int)nbf, na, c->Be, c->segs, c->nseg, c->Bs, c->splane, s));
+          if (c->Bd)
+            HIPCHK(nopt::split_packed_i8(dA, w, m0, m1, (int)nbf, na, c->Be, 4, c->Bd, c->ld8, rows * c->ld8, s));
+        }
         m0 = m1;
       }
       if (rmax && pass == 0) HIPCHK(nopt::rowmax_exponents(rmax, rows, c->Be, s));
     }
-    if (keep64 && keep8)
-      HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)nbf, rows, 5, c->Bd, c->ld8, rows, rows * c->ld8, c->Be, s));
-    if (keep8)
-      HIPCHK(nopt::pack_segments(c->Bd, rows * c->ld8, c->ld8, (int)nbf, na, c->segs, c->nseg, c->Bs, c->splane, s));
+    if (keep64 && keep8) {
+      // the exponents (and digit planes) from the FP64 rows, then the packed triangle
+      HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)nbf, rows, c->Bd ? 4 : 0, c->Bd, c->ld8, rows,
+                                 rows * c->ld8, c->Be, s));
+      HIPCHK(nopt::pack_segs_rows(c->B64, c->ld64, (int)nbf, na, c->Be, c->segs, c->nseg, c->Bs, c->splane, s));
+    }
+    if (keep8) HIPCHK(nopt::expo_bytes(c->Be, (int)nbf, c->ld8, na, c->Be8, s));
     HIPCHK(hipStreamSynchronize(s));
     return NOPT_OK;
   };
@@ -997,8 +1011,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
   const bool f64 = prec == NOPT_PREC_FP64, i8 = prec == NOPT_PREC_INT8;
   const bool bf = prec == NOPT_PREC_BF16X3 || prec == NOPT_PREC_BF16X6;
-  if (i8 && !c->Bd)
-    return set_error(NOPT_ESTATE, "the context keeps no digit planes of B (NOPT_CTX_FP64_ONLY): integer mode unavailable");
+  if (i8 && !c->Bs)
+    return set_error(NOPT_ESTATE, "the context keeps no integer form of B (NOPT_CTX_FP64_ONLY): integer mode unavailable");
   if (!i8 && !c->B64)
     return set_error(NOPT_ESTATE, "the context keeps no FP64 B (NOPT_CTX_INT8_ONLY): only the integer mode is available");
   const long unit = bf ? 32 : 16;  // column padding of each orbital block
@@ -1029,21 +1043,44 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   const long rows = c->naux * c->nbf;
   const long nn = (long)nbf * nbf;
   int rc;
-  // integer mode with one column group: the half-transform writes the digit
-  // planes of X directly (per density), X itself is never stored; pairs of
+  // integer mode: the half-transform writes the digit planes of X directly,
+  // X itself is never stored. Each density's orbital columns are cut at the
+  // column groups into segments (at most nopt::kMaxXSeg in all), and each
+  // segment's rows get their own exponent and digit planes; pairs of
   // auxiliary functions share the exponent of a row m and lie side by side
-  // (npq rows of pairs), so the exchange streams 2*nocc orbitals per exponent.
-  // (Triples of functions pad less for 81 orbitals, 256 columns per 3 against
-  // 192 per 2, but their half-transform, one block of 12 waves per CU, was
-  // slower than the exchange gained: 535 against 557 builds/s at C20H42.)
-  const bool xfused = i8 && anyK && ngroups == 1;
+  // (npq rows of pairs), so the exchange streams 2*len orbitals per exponent,
+  // one launch per segment into its own range of K partials. (Triples of
+  // functions pad less for 81 orbitals, 256 columns per 3 against 192 per 2,
+  // but their half-transform, one block of 12 waves per CU, was slower than
+  // the exchange gained: 535 against 557 builds/s at C20H42.)
+  const long bn = 16L * nt;
+  int nxs = 0, sfirst[kMaxBatch + 1];  // segments of density d: sfirst[d] .. sfirst[d+1]-1
+  struct Seg { int off, len; };
+  Seg segl[nopt::kMaxXSeg];
+  bool xfits = true;
+  for (int d = 0; d < nd; ++d) {
+    sfirst[d] = nxs;
+    if (off[d + 1] == off[d]) continue;
+    const long a = off[d], b = off[d] + it[d].nocc;
+    for (long g = a / bn; g * bn < b; ++g) {
+      const long s0 = std::max(a, g * bn), s1 = std::min(b, (g + 1) * bn);
+      if (nxs == nopt::kMaxXSeg) {
+        xfits = false;
+        break;
+      }
+      segl[nxs++] = Seg{(int)s0, (int)(s1 - s0)};
+    }
+  }
+  sfirst[nd] = nxs;
+  const bool xfused = i8 && anyK && xfits;
   const int qpb = xfused ? 2 : 1;
   const long npq = (naux + qpb - 1) / qpb, rowsq = npq * nbf;
-  auto ldk_of = [&](long nocc) { return xfused ? round_up(qpb * nocc, 64) : round_up(nocc, 64); };
-  long xoff[kMaxBatch + 1];  // digit-plane offsets (bytes) of each density
+  auto ldk_of = [&](long len) { return xfused ? round_up(qpb * len, 64) : round_up(len, 64); };
+  long xoff[nopt::kMaxXSeg + 1];  // digit-plane offsets (bytes) of each segment
   xoff[0] = 0;
-  for (int d = 0; d < nd; ++d)
-    xoff[d + 1] = xoff[d] + ((off[d + 1] > off[d]) ? 4 * (xfused ? rowsq : rows) * ldk_of(it[d].nocc) : 0);
+  for (int k = 0; k < (xfused ? nxs : 0); ++k) xoff[k + 1] = xoff[k] + 4 * rowsq * ldk_of(segl[k].len);
+  int segs_max = 1;  // most segments of one density (K partial ranges)
+  for (int d = 0; d < nd && xfused; ++d) segs_max = std::max(segs_max, sfirst[d + 1] - sfirst[d]);
   // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
   const int nsj_min = !nj ? 0 : (naux + JCHUNK_MAX / nj - 1) / (JCHUNK_MAX / nj);
   const int nsj = !nj ? 0 : i8 ? c->nsj8 : std::max(nsj_min, c->nsj > 0 ? c->nsj : NSPLIT_J_MIN);
@@ -1058,13 +1095,13 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       long ldk_max = 0;
       for (int d = 0; d < nd; ++d)
         if (off[d + 1] > off[d]) ldk_max = std::max(ldk_max, ldk_of(it[d].nocc));
-      // fused: planes of every density side by side; otherwise one density
+      // fused: planes of every segment side by side; otherwise one density
       // at a time
-      wb.Xd = xfused ? (size_t)xoff[nd] : (size_t)4 * rows * ldk_max;
-      wb.Xe = sizeof(int) * (xfused ? rowsq * nd : rows);
+      wb.Xd = xfused ? (size_t)xoff[nxs] : (size_t)4 * rows * ldk_max;
+      wb.Xe = sizeof(int) * (xfused ? rowsq * nxs : rows);
     }
   }
-  wb.Kpart = sizeof(double) * (i8 ? (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
+  wb.Kpart = sizeof(double) * (i8 ? (size_t)segs_max * c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
                                   : (size_t)c->nsk * c->ntiles * KT * KT);
   if (nj) wb.Jpart = sizeof(double) * nj * nsj * nn;
   wb.Dp = sizeof(double) * std::max(nj, 1) * c->dstride;
@@ -1175,21 +1212,20 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (i8) {
       nopt::XSplit xs{};
       if (xfused)
-        for (int d = 0; d < nd; ++d) {
-          if (off[d + 1] == off[d]) continue;
-          const int k = xs.nseg++;
-          const long ldk = ldk_of(it[d].nocc);
-          xs.off[k] = (int)off[d];
-          xs.len[k] = (int)it[d].nocc;
+        for (int k = 0; k < nxs; ++k) {
+          const long ldk = ldk_of(segl[k].len);
+          xs.off[k] = segl[k].off;
+          xs.len[k] = segl[k].len;
           xs.ldk[k] = ldk;
           xs.xplane[k] = rowsq * ldk;
-          xs.xd[k] = (int8_t*)w.Xd + xoff[d];
-          xs.xe[k] = (int*)w.Xe + (long)d * rowsq;
+          xs.xd[k] = (int8_t*)w.Xd + xoff[k];
+          xs.xe[k] = (int*)w.Xe + (long)k * rowsq;
+          xs.nseg = k + 1;
         }
       if (xfused) gh.y = (unsigned)npq;
-      e = nopt::launch_half_i8(nt, qpb, gh, s, c->Bd, rows * c->ld8, c->Be, c->ld8, nbf, naux,
-                               (const int8_t*)w.Cd, ldx * c->ld8, (const int*)w.Ce,
-                               xfused ? nullptr : (double*)w.X, ldx, xs);
+      const nopt::HalfB hb{c->Bd, rows * c->ld8, reinterpret_cast<const int*>(c->Bs), 16L * c->nseg, c->Be8, c->Be};
+      e = nopt::launch_half_i8(nt, qpb, gh, s, hb, c->ld8, nbf, naux, (const int8_t*)w.Cd, ldx * c->ld8,
+                               (const int*)w.Ce, xfused ? nullptr : (double*)w.X, ldx, xs);
     }
     else if (f64)
       e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)w.Cp, w.Dp, (double*)w.X, ldx,
@@ -1221,18 +1257,29 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     const unsigned nel = (unsigned)((nn + 255) / 256);
     if (i8) {
-      const long ldk = ldk_of(it[d].nocc);
-      const int8_t* xd = (const int8_t*)w.Xd + (xfused ? xoff[d] : 0);
-      const int* xe = (const int*)w.Xe + (xfused ? (long)d * rowsq : 0);
-      if (!xfused)
-        HIPCHK(nopt::split_rows_i8((const double*)w.X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)xd, ldk,
-                                   rows, rows * ldk, (int*)xe, s));
       // the exchange runs over the rows of the digit planes: groups of qpb
-      // functions (npq of them) when fused, single functions otherwise
-      const long nq = xfused ? npq : naux;
-      HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, nq * nbf * ldk, xe, ldk, nbf, (int)nq, w.Kpart));
-      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf, nullptr,
-                                     it[d].K);
+      // functions (npq of them) per segment when fused, single functions of
+      // the whole density otherwise; each segment fills its own nsk8 ranges
+      // of K partials and the finalize sums them all
+      const size_t kseg = (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile;
+      int nsum = 1;
+      if (xfused) {
+        nsum = sfirst[d + 1] - sfirst[d];
+        for (int k = sfirst[d]; k < sfirst[d + 1]; ++k) {
+          const long ldk = ldk_of(segl[k].len);
+          HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, (const int8_t*)w.Xd + xoff[k], rowsq * ldk,
+                                      (const int*)w.Xe + (long)k * rowsq, ldk, nbf, (int)npq,
+                                      w.Kpart + (k - sfirst[d]) * kseg));
+        }
+      } else {
+        const long ldk = ldk_of(it[d].nocc);
+        HIPCHK(nopt::split_rows_i8((const double*)w.X + off[d], ldx, (int)it[d].nocc, rows, 4, (int8_t*)w.Xd, ldk,
+                                   rows, rows * ldk, (int*)w.Xe, s));
+        HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, (const int8_t*)w.Xd, rows * ldk, (const int*)w.Xe, ldk,
+                                    nbf, (int)naux, w.Kpart));
+      }
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, nsum * c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf,
+                                     nullptr, it[d].K);
     } else {
       dim3 gk(c->ntiles, c->nsk);
       if (f64)
@@ -1315,8 +1362,12 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   nopt::clear_error();
   if (!out) return set_error(NOPT_EINVAL, "nopt_jk_create: null output pointer");
   *out = nullptr;
-  if (flags != NOPT_CTX_DEFAULT && flags != NOPT_CTX_FP64_ONLY && flags != NOPT_CTX_INT8_ONLY)
-    return set_error(NOPT_EINVAL, "nopt_jk_create_ex: flags must be one of NOPT_CTX_DEFAULT, _FP64_ONLY, _INT8_ONLY");
+  const int form = flags & ~NOPT_CTX_COMPACT;
+  if ((form != NOPT_CTX_DEFAULT && form != NOPT_CTX_FP64_ONLY && form != NOPT_CTX_INT8_ONLY) ||
+      (form == NOPT_CTX_FP64_ONLY && (flags & NOPT_CTX_COMPACT)))
+    return set_error(NOPT_EINVAL,
+                     "nopt_jk_create_ex: flags must be one of NOPT_CTX_DEFAULT, _FP64_ONLY, _INT8_ONLY, the first "
+                     "or last with NOPT_CTX_COMPACT");
   if (nbf <= 0 || naux <= 0) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf and naux must be positive");
   if (nbf > 2046) return set_error(NOPT_EINVAL, "nopt_jk_create: nbf above 2046 is not supported");
   if (naux > 65535) return set_error(NOPT_EINVAL, "nopt_jk_create: naux above 65535 is not supported");
@@ -1334,6 +1385,8 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   c->ld64 = round_up(nbf + 1, 16);  // ld > nbf: see k_jpart
   c->mtiles = (int)((nbf + BM - 1) / BM);
   c->ld8 = round_up(nbf, 64);  // int8 MFMA k-steps of 64
+  // the full-row digit planes of the half-transform, unless large or declined
+  c->bfull = !(flags & NOPT_CTX_COMPACT) && 4.0 * naux * nbf * c->ld8 <= (double)kFullRowsMax;
   c->mtiles64 = (int)((nbf + 63) / 64);
   c->ktn = (int)((nbf + KT - 1) / KT);
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
@@ -1451,6 +1504,7 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->info);
   dev_free(c->Bd);
   dev_free(c->Be);
+  dev_free(c->Be8);
   dev_free(c->Cd);
   dev_free(c->Ce);
   dev_free(c->Xd);
@@ -1843,7 +1897,8 @@ long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   if (!c) return 0;
   long b = 0;
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
-  if (c->Bd) b += 5L * c->naux * c->nbf * c->ld8 + sizeof(int) * c->naux * c->nbf + 5L * c->splane;
+  if (c->Bd) b += 4L * c->naux * c->nbf * c->ld8;
+  if (c->Bs) b += sizeof(int) * c->naux * c->nbf + c->naux * c->ld8 + 5L * c->splane;
   b += (long)(c->X_bytes + c->Cp_bytes + c->Cd_bytes + c->Ce_bytes + c->Xd_bytes + c->Xe_bytes + c->Jpart_bytes);
   b += (long)c->Kpart_bytes;
   b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux + kMaxBatch * c->dstride + 5 * c->nbf * c->nbf);

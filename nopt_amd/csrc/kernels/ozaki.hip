@@ -44,12 +44,31 @@ __device__ __forceinline__ void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// row maximum -> exponent e with |x| 2^-e < 1 (0 for an all-zero row)
+// row maximum -> exponent e with |x| 2^-e < 1 (0 for an all-zero row); at
+// least kMinExp, so that every exponent fits a byte (a row below 2^-121 keeps
+// its digits at the scale 2^-120: absolute resolution 2^-147)
+constexpr int kMinExp = -120;
 __device__ __forceinline__ int row_exponent(double mx) {
   if (!(mx > 0.0)) return 0;
   int e;
   (void)frexp(mx, &e);  // mx = f 2^e, f in [0.5, 1)
-  return e;
+  return max(e, kMinExp);
+}
+
+// The five balanced base-128 digits of x 2^(6-e) as k_split_rows forms them,
+// returned as N28 = sum_{k<4} d_k 2^(7(3-k)) (the first four digits, |N28| <=
+// 2^27) and d4: N35 = 128 N28 + d4 = rint(x 2^(34-e)).
+__device__ __forceinline__ void digits5(double x, int e, int& n28, int& d4) {
+  double r = ldexp(x, 6 - e);
+  int n = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double d = rint(r);
+    n = n * 128 + (int)d;
+    r = (r - d) * 128.0;
+  }
+  n28 = n;
+  d4 = (int)rint(r);
 }
 
 // Splits the rows of a row-major FP64 matrix (rows_src x cols, ld) into ns
@@ -162,6 +181,64 @@ __global__ void k_rowmax_exponent(const unsigned long long* __restrict__ rowmax,
   if (r < rows) expo[r] = row_exponent(__longlong_as_double((long long)rowmax[r]));
 }
 
+// ------------------------------------------------ packed lower triangle of B
+// B's lower triangle as packed rows cut into segments of 16 columns: segment
+// s = (row m, columns n0 .. n0+15), n0 a multiple of 16, with the digits of
+// row m (exponent Be[Q*nbf + m]) and zeros beyond column m. Row m holds
+// m/16 + 1 segments, starting at segment seg0(m) = sum_{r<m} (r/16 + 1), and
+// the segments of all rows follow each other, so one B_Q is a contiguous run
+// of sq = 16*nseg values and the element (Q, m, n <= m) sits at
+// e = Q*sq + 16*seg0(m) + n. seg_code[s] = (m << 16) | n0. A value is kept as
+// its 35-bit integer N35 = rint(B 2^(34-e)) = 128 N28 + d4 (digits5): the
+// int32 N28, the value to 28 bits, at ((int*)Bs)[e] (the first 4*splane
+// bytes), and the signed byte d4 at Bs[4*splane + e]. The Coulomb kernels
+// rebuild N35 exactly as fma(N28, 128, d4); the half-transform reads N28
+// alone.
+__host__ __device__ __forceinline__ long seg0(int m) {
+  const long a = m / 16, b = m % 16;
+  return m + 8 * a * (a - 1) + a * b;
+}
+
+// Bs for the segments [s0, s1) (the rows of one chunk) with the row exponents
+// Be: SRC 0 from FP64 rows B[(Q*nbf + m)*ld + n], SRC 1 from a packed chunk
+// A[Q*cp + m(m+1)/2 - m0(m0+1)/2 + n] of the rows m0 <= m < m1. Grid:
+// (s1 - s0)/256 x naux, one segment per thread.
+template <int SRC>
+__global__ __launch_bounds__(256) void k_pack_segs(const double* __restrict__ A, long ld, int m0, int nbf,
+                                                   const int* __restrict__ Be, const int* __restrict__ seg_code,
+                                                   int s0, int s1, long sq, int8_t* __restrict__ Bs, long splane) {
+  const int s = s0 + blockIdx.x * 256 + threadIdx.x;
+  if (s >= s1) return;
+  const long Q = blockIdx.y;
+  const int code = seg_code[s], m = code >> 16, n0 = code & 0xffff;
+  const int e = Be[Q * nbf + m];
+  const double* src = SRC == 0 ? A + (Q * nbf + m) * ld + n0
+                               : A + Q * ld + (long)m * (m + 1) / 2 - (long)m0 * (m0 + 1) / 2 + n0;
+  int part[16];
+  uint32_t low[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    int n28 = 0, d4 = 0;
+    if (n0 + j <= m) digits5(src[j], e, n28, d4);
+    part[j] = n28;
+    low[j / 4] |= (uint32_t)(d4 & 255) << (8 * (j % 4));
+  }
+  const long el = Q * sq + 16L * s;  // element of the first column
+  uint4* p32 = reinterpret_cast<uint4*>(reinterpret_cast<int*>(Bs) + el);
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    p32[c] = uint4{(uint32_t)part[4 * c], (uint32_t)part[4 * c + 1], (uint32_t)part[4 * c + 2], (uint32_t)part[4 * c + 3]};
+  *reinterpret_cast<uint4*>(Bs + 4 * splane + el) = uint4{low[0], low[1], low[2], low[3]};
+}
+
+// the row exponents as bytes, Be8[Q*ld8 + n] (zero padding to ld8)
+__global__ void k_expo8(const int* __restrict__ Be, int nbf, long ld8, long naux, int8_t* __restrict__ Be8) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= naux * ld8) return;
+  const long Q = i / ld8, n = i % ld8;
+  Be8[i] = n < nbf ? (int8_t)Be[Q * nbf + n] : (int8_t)0;
+}
+
 // ------------------------------------------------------------ half-transform
 // LDS position of 16-byte chunk c (of a 64-byte k-slab) in row r: an XOR
 // swizzle under which the 16 rows x 4 chunks of one MFMA operand read, and
@@ -183,9 +260,20 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 // written side by side, so that the exchange reads 2 x len orbitals per
 // exponent (Q + 1 = naux contributes zeros). BMI rows per tile (64, or 32 for
 // two functions, so that two blocks still share a CU); WPE waves per SIMD.
-template <int NT, int WC, int NW, int QPB, int BMI, int WPE>
+// PK reads B from the packed lower triangle instead (Bp = the N28 of Bs, sq
+// values per function, Be8 the row exponents as bytes, row stride ld8): an
+// element (m, n) with n <= m is N28 of row m as stored; one with n > m is
+// the stored (n, m), N28 at the scale of row n, brought to the scale of row
+// m by a power of two (rounded to nearest on a right shift). Each thread
+// turns four consecutive columns of a row into one dword per digit plane. In
+// a slab left of the tile's diagonal block the lanes walk rows of the stored
+// triangle (16-byte loads); right of it and on it the lanes walk the tile's
+// rows, and each loads one column of four stored rows (the lanes of a
+// function read contiguous dwords).
+template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
 __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_half_i8(
-    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Be, long ld8, int nbf, int naux,
+    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Bp, long sq, const int8_t* __restrict__ Be8,
+    const int* __restrict__ Be, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
   constexpr int BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
@@ -212,12 +300,47 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   const int ar = (t >> 2) % BMI, ac = t & 3;
   const int8_t* ap[APT];
   bool aok[APT];
+  if constexpr (!PK) {
 #pragma unroll
-  for (int u = 0; u < APT; ++u) {
-    const int e = t + NTH * u, qe = e / ACQ, pe = (e % ACQ) / (4 * BMI);
-    aok[u] = m0 + ar < nbf && Qb + qe < naux;
-    ap[u] = Bd + ((Qb + qe) * nbf + m0 + ar) * ld8 + 16 * ac + pe * bplane;
+    for (int u = 0; u < APT; ++u) {
+      const int e = t + NTH * u, qe = e / ACQ, pe = (e % ACQ) / (4 * BMI);
+      aok[u] = m0 + ar < nbf && Qb + qe < naux;
+      ap[u] = Bd + ((Qb + qe) * nbf + m0 + ar) * ld8 + 16 * ac + pe * bplane;
+    }
   }
+  // packed loader: GPT groups of four columns per thread and slab. Row-walk
+  // (slabs left of the diagonal block): group e = t + NTH u is columns
+  // 4 (e % 16) .. +3 of tile row (e / 16) % BMI of function e / (16 BMI).
+  // Column-walk: lane l takes tile row l % BMI of function l / BMI, and the
+  // wave's group index is w + NWT u (columns 4 (w + NWT u) .. +3).
+  constexpr int GPT = QPB * BMI * 16 / NTH;
+  static_assert(!PK || (GPT * NTH == QPB * BMI * 16 && QPB * BMI == 64 && NWT * GPT == 16), "packed loader shape");
+  const int tq = lane / BMI, tr = lane % BMI;  // column-walk: function and tile row of the lane
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  // buffer resources from the block's first function (int32 byte offsets;
+  // the ranges end with the last function, so loads beyond read zeros): the
+  // N28 values, and the exponent bytes
+  const long nfq = PK ? min((long)QPB, naux - Qb) : 0;  // functions of this block
+  const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<int*>(Bp + (PK ? Qb * sq : 0)), (short)0, (int)(4 * sq * nfq), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rE = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<int8_t*>(Be8 + (PK ? Qb * ld8 : 0)), (short)0, (int)(ld8 * nfq), 0x00020000);
+  const int tqs = tq * (int)sq, tqe = tq * (int)ld8;
+  const bool tok = PK && m0 + tr < nbf && Qb + tq < naux;
+  const int em = tok ? (int)(int8_t)__builtin_amdgcn_raw_buffer_load_b8(rE, tqe + m0 + tr, 0, 0) : 0;
+  int doff[PK ? GPT : 1];
+  bool dok[PK ? GPT : 1];
+  if constexpr (PK) {
+#pragma unroll
+    for (int u = 0; u < GPT; ++u) {
+      const int e = t + NTH * u, q = e / (16 * BMI), r = (e / 16) % BMI, g = e % 16;
+      dok[u] = m0 + r < nbf && Qb + q < naux;
+      doff[u] = q * (int)sq + 16 * (int)seg0(dok[u] ? m0 + r : 0) + 4 * g;
+    }
+  }
+  uint4 praw[PK ? GPT : 1];   // N28 of the group's four columns
+  uint32_t pexp[PK ? GPT : 1];  // column-walk: exponent bytes of the four stored rows
+  int pn0 = 0, ptype = 0;       // slab of the prefetched values and its walk (0 rows, 1 columns, 2 diagonal block)
   const int8_t* Cbase = Cd + (long)cg * BN * ld8;
   // C loader: chunk e = t + NTH u < CCH of the slab is chunk (e & 3) of column
   // (e >> 2) % BN in plane e / (4 BN); global and LDS offsets per u
@@ -236,10 +359,99 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   constexpr int CH = (CPT + 1) / 2;
   uint4 a_reg[APT], c_lo[CH], c_hi[CPT - CH > 0 ? CPT - CH : 1];
 
+  // packed slab at columns n0..n0+63: raw N28 (and exponent bytes) into registers
+  auto gload_packed = [&](int n0) __attribute__((always_inline)) {
+    pn0 = n0;
+    ptype = n0 + KS - 1 <= m0 ? 0 : n0 >= m0 + BMI - 1 ? 1 : 2;
+    if (ptype == 0) {
+#pragma unroll
+      for (int u = 0; u < GPT; ++u)
+        if (dok[u]) {
+          const auto x = __builtin_amdgcn_raw_buffer_load_b128(rB, 4 * (doff[u] + n0), 0, 0);
+          praw[u] = uint4{x[0], x[1], x[2], x[3]};
+        } else {
+          praw[u] = uint4{0, 0, 0, 0};
+        }
+      return;
+    }
+    const int m = m0 + tr;
+#pragma unroll
+    for (int u = 0; u < GPT; ++u) {
+      const int n4 = n0 + 4 * (wu + NWT * u);  // wave-uniform
+      pexp[u] = tok && n4 < nbf ? __builtin_amdgcn_raw_buffer_load_b32(rE, tqe, n4, 0) : 0u;
+      uint32_t v[4];
+      if (ptype == 1) {
+        // every stored row n >= m: the row offset is wave-uniform (scalar)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n4 + j;
+          const int so = __builtin_amdgcn_readfirstlane(64 * (int)seg0(min(n, nbf - 1)));
+          v[j] = tok && n < nbf ? __builtin_amdgcn_raw_buffer_load_b32(rB, 4 * (tqs + m), so, 0) : 0u;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = n4 + j;
+          const int r = max(m, n), c = min(m, n);
+          v[j] = tok && n < nbf ? __builtin_amdgcn_raw_buffer_load_b32(rB, 4 * (tqs + 16 * (int)seg0(r) + c), 0, 0)
+                                : 0u;
+        }
+      }
+      praw[u] = uint4{v[0], v[1], v[2], v[3]};
+    }
+  };
+  // the prefetched slab as four digit planes in LDS (one dword per plane and group)
+  auto lstore_packed = [&](int8_t* As) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < GPT; ++u) {
+      int q, r, g;
+      int v[4] = {(int)praw[u].x, (int)praw[u].y, (int)praw[u].z, (int)praw[u].w};
+      if (ptype == 0) {
+        const int e = t + NTH * u;
+        q = e / (16 * BMI), r = (e / 16) % BMI, g = e % 16;
+      } else {
+        q = tq, r = tr, g = wu + NWT * u;
+        const int m = m0 + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int n = pn0 + 4 * g + j;
+          // stored at row n (scale e_n) when n > m: times 2^(e_n - e_m)
+          // (right of the diagonal block n >= m, and e_n = e_m for n = m)
+          const int d = (ptype == 1 || n > m) ? (int)(int8_t)(pexp[u] >> (8 * j)) - em : 0;
+          const int k = min(max(-d, 0), 31);
+          v[j] = d > 0 ? (int)((uint32_t)v[j] << d) : (v[j] + (int)((1u << k) >> 1)) >> k;
+        }
+      }
+      // balanced digits of each value (bytes d0..d3 of one dword), then a
+      // 4 x 4 byte transpose into one dword per plane
+      uint32_t f[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int M = v[j] + 1056832;
+        const uint32_t uu = (((uint32_t)M >> 14 & 127u) << 8) | (((uint32_t)M >> 7 & 127u) << 16) |
+                            (((uint32_t)M & 127u) << 24);
+        const uint32_t ff = uu ^ 0x40404000u;
+        f[j] = ff | ((ff & 0x40404000u) << 1) | ((uint32_t)(M >> 21) & 255u);
+      }
+      const uint32_t lo01 = __builtin_amdgcn_perm(f[1], f[0], 0x05010400u);
+      const uint32_t hi01 = __builtin_amdgcn_perm(f[1], f[0], 0x07030602u);
+      const uint32_t lo23 = __builtin_amdgcn_perm(f[3], f[2], 0x05010400u);
+      const uint32_t hi23 = __builtin_amdgcn_perm(f[3], f[2], 0x07030602u);
+      const uint32_t pl[4] = {__builtin_amdgcn_perm(lo23, lo01, 0x05040100u),
+                              __builtin_amdgcn_perm(lo23, lo01, 0x07060302u),
+                              __builtin_amdgcn_perm(hi23, hi01, 0x05040100u),
+                              __builtin_amdgcn_perm(hi23, hi01, 0x07060302u)};
+      int8_t* base = As + q * AB + r * KS + 16 * swz(r, g >> 2) + 4 * (g & 3);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) *reinterpret_cast<uint32_t*>(base + p * BMI * KS) = pl[p];
+    }
+  };
 #define NOPT_GLOAD_A(k0)                                                                                 \
   do {                                                                                                   \
-    _Pragma("unroll") for (int u = 0; u < APT; ++u) a_reg[u] =                                           \
-        aok[u] ? *reinterpret_cast<const uint4*>(ap[u] + (k0)) : uint4{0, 0, 0, 0};                       \
+    if constexpr (PK) gload_packed(k0);                                                                  \
+    else                                                                                                 \
+      _Pragma("unroll") for (int u = 0; u < APT; ++u) a_reg[u] =                                         \
+          aok[u] ? *reinterpret_cast<const uint4*>(ap[u] + (k0)) : uint4{0, 0, 0, 0};                     \
   } while (0)
 #define NOPT_GLOAD_C(k0)                                                                                 \
   do {                                                                                                   \
@@ -254,11 +466,12 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
 #define NOPT_LSTORE(buf)                                                                                 \
   do {                                                                                                   \
     int8_t* As_ = sm[buf];                                                                               \
-    _Pragma("unroll") for (int u = 0; u < APT; ++u) {                                                    \
-      const int e_ = t + NTH * u;                                                                        \
-      *reinterpret_cast<uint4*>(As_ + (e_ / ACQ) * AB + (((e_ % ACQ) / (4 * BMI)) * BMI + ar) * KS +     \
-                                16 * swz(ar, ac)) = a_reg[u];                                            \
-    }                                                                                                    \
+    if constexpr (!PK)                                                                                   \
+      _Pragma("unroll") for (int u = 0; u < APT; ++u) {                                                  \
+        const int e_ = t + NTH * u;                                                                      \
+        *reinterpret_cast<uint4*>(As_ + (e_ / ACQ) * AB + (((e_ % ACQ) / (4 * BMI)) * BMI + ar) * KS +   \
+                                  16 * swz(ar, ac)) = a_reg[u];                                          \
+      }                                                                                                  \
     static_for<CPT>([&](auto U) {                                                                        \
       constexpr int u_ = decltype(U)::value;                                                             \
       uint4* d_ = reinterpret_cast<uint4*>(As_ + c_s[u_]);                                               \
@@ -277,6 +490,7 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
       for (int c = 0; c < 5; ++c) acc[i][j][c] = v4i_t{0, 0, 0, 0};
   const int nks = (int)(ld8 / KS);
   NOPT_GLOAD_A(0);
+  if constexpr (PK) lstore_packed(sm[0]);
   NOPT_GLOAD_C(0);
   NOPT_LSTORE(0);
   __syncthreads();
@@ -335,6 +549,9 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
       }
     }
     if (ks + 1 < nks) {
+      // packed: the B slab is converted and stored before the C slab is
+      // fetched (fewer registers live at once)
+      if constexpr (PK) lstore_packed(sm[cur ^ 1]);
       NOPT_GLOAD_C((ks + 1) * KS);
       NOPT_LSTORE(cur ^ 1);
     }
@@ -389,7 +606,8 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
       }
     return;
   }
-  // fused digit split (one column group): per segment the row exponent (the
+  // fused digit split (the segments inside the block's column group): per
+  // segment the row exponent (the
   // largest frexp exponent over the segment's columns: 16 lanes of a wave,
   // then the WC waves of a row and the QPB functions of the block via LDS),
   // then each value as the integer N = rint(x 2^(27-e)), |N| <= 2^27, in
@@ -411,7 +629,8 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   int(*rexp)[BMI] = reinterpret_cast<int(*)[BMI]>(&sm[1][0]);
   const long orow = (long)blockIdx.y * nbf;  // first output row of the block
   for (int d = 0; d < xs.nseg; ++d) {
-    const int o = xs.off[d], len = xs.len[d], pad = (len + 15) & ~15;
+    if (xs.off[d] / BN != cg) continue;  // a segment of another column group (uniform per block)
+    const int o = xs.off[d] - cg * BN, len = xs.len[d], pad = (len + 15) & ~15;  // o: within the group
     const int ldk = (int)xs.ldk[d], srow = ldk + 4;
     const int nch = BMI * srow / 4;  // 16-byte chunks of the staging tile
     for (int i = t; i < nch; i += NTH) reinterpret_cast<uint4*>(st)[i] = uint4{0, 0, 0, 0};
@@ -503,64 +722,13 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
 }
 
 // ------------------------------------------------- Coulomb over packed rows
-// The Coulomb pair reads B's lower triangle as packed rows cut into segments
-// of 16 columns: segment s = (row m, columns n0 .. n0+15), n0 a multiple of
-// 16, with the digits of row m (exponent Be[Q*nbf + m]) and zeros beyond
-// column m. Row m holds m/16 + 1 segments, starting at segment
-// seg0(m) = sum_{r<m} (r/16 + 1), and the segments of all rows follow each
-// other, so one B_Q is a contiguous run of sq = 16*nseg values.
-// seg_code[s] = (m << 16) | n0. A value is stored as the 35-bit integer
-// N = sum_k d_k 2^(7(4-k)) of its five digits (B = N 2^(e-34)), split into
-// an int32 part N >> 8 and an unsigned byte N & 255: the int32 parts at
-// element Q*sq + 16*s + j of Bs read as int32 (the first 4*splane bytes), the
-// bytes at Bs[4*splane + Q*sq + 16*s + j]. A kernel rebuilds N exactly as
-// fma(part, 256, byte), two conversions instead of five digit extractions.
-__host__ __device__ __forceinline__ long seg0(int m) {
-  const long a = m / 16, b = m % 16;
-  return m + 8 * a * (a - 1) + a * b;
-}
-
-// Bs from the full digit rows: segment s of B_Q formed from row (Q, m),
-// values beyond column m cleared. Grid: segments / 256 x naux.
-__global__ __launch_bounds__(256) void k_pack_segments(const int8_t* __restrict__ Bd, long bplane, long ld8, int nbf,
-                                                       const int* __restrict__ seg_code, int nseg,
-                                                       int8_t* __restrict__ Bs, long splane) {
-  const int s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= nseg) return;
-  const long Q = blockIdx.y;
-  const int code = seg_code[s], m = code >> 16, n0 = code & 0xffff;
-  const int valid = m - n0 + 1;  // columns of the row in this segment (1 .. 16)
-  const int8_t* src = Bd + (Q * nbf + m) * ld8 + n0;
-  uint4 dg[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) dg[k] = *reinterpret_cast<const uint4*>(src + k * bplane);
-  const int8_t* d = reinterpret_cast<const int8_t*>(dg);
-  int part[16];
-  uint32_t low[4] = {0, 0, 0, 0};
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    long n = 0;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) n = n * 128 + d[16 * k + j];
-    if (j >= valid) n = 0;
-    part[j] = (int)(n >> 8);
-    low[j / 4] |= (uint32_t)(n & 255) << (8 * (j % 4));
-  }
-  const long e = Q * 16L * nseg + 16L * s;  // element of the first column
-  uint4* p32 = reinterpret_cast<uint4*>(reinterpret_cast<int*>(Bs) + e);
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-    p32[c] = uint4{(uint32_t)part[4 * c], (uint32_t)part[4 * c + 1], (uint32_t)part[4 * c + 2], (uint32_t)part[4 * c + 3]};
-  *reinterpret_cast<uint4*>(Bs + 4 * splane + e) = uint4{low[0], low[1], low[2], low[3]};
-}
-
 // The lower triangle of every B_Q is cut into 8-column slots (m, n0), n0 <= m,
 // listed in `slots` as (m << 16) | n0. Both Coulomb kernels read the int32
 // parts and the bytes of a slot (two 16-byte and one 8-byte load) and
-// rebuild the integers N = 256 part + byte exactly in FP64.
+// rebuild the integers N35 = 128 N28 + d4 exactly in FP64.
 struct Dig8 {
-  uint4 p[2];  // int32 parts of the 8 columns
-  uint2 b;     // their low bytes
+  uint4 p[2];  // N28 of the 8 columns
+  uint2 b;     // their last digits d4
 };
 // element e = Q*sq + 16*seg0(m) + n0 of the packed layout
 __device__ __forceinline__ void load_dig8(const int8_t* Bs, long splane, long e, Dig8& d) {
@@ -582,9 +750,9 @@ __device__ __forceinline__ void load_dig16(const int8_t* Bs, long splane, long e
 }
 __device__ __forceinline__ void unpack8(const Dig8& d, double (&v)[8]) {
   const int* q = reinterpret_cast<const int*>(d.p);
-  const uint8_t* b = reinterpret_cast<const uint8_t*>(&d.b);
+  const int8_t* b = reinterpret_cast<const int8_t*>(&d.b);
 #pragma unroll
-  for (int j = 0; j < 8; ++j) v[j] = fma((double)q[j], 256.0, (double)b[j]);
+  for (int j = 0; j < 8; ++j) v[j] = fma((double)q[j], 128.0, (double)b[j]);
 }
 
 // Sum over the 64 lanes of a wave, valid in every lane: rotations within each
@@ -1048,32 +1216,36 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
   return hipGetLastError();
 }
 
-hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                          long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
-                          long ldx, const XSplit& xs) {
+hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB& b, long ld8, int nbf, int naux,
+                          const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx, const XSplit& xs) {
   // ld8 <= 2048 keeps 128 A0 + A1 exact in int32 (the epilogue's class combine)
   if (ld8 % 64 != 0 || ld8 > 2048 || (long)grid.z * 16 * nt > ldx || qpb < 1 || qpb > 2)
     return hipErrorInvalidValue;
   if ((long)grid.y * qpb < naux || (long)(grid.y - 1) * qpb >= naux) return hipErrorInvalidValue;
-  if (xs.nseg < 0 || xs.nseg > 4 || (xs.nseg > 0 && grid.z != 1)) return hipErrorInvalidValue;
-  const long ldk_max = (qpb * 16 * nt + 63) / 64 * 64;
-  for (int d = 0; d < xs.nseg; ++d)
-    if (xs.off[d] % 16 || xs.len[d] <= 0 || xs.ldk[d] % 64 || xs.ldk[d] > ldk_max ||
-        xs.ldk[d] < (long)qpb * xs.len[d] || xs.off[d] + ((xs.len[d] + 15) & ~15) > 16 * nt || !xs.xd[d] ||
-        !xs.xe[d])
+  if (!b.Be || (!b.Bd && (!b.Bp || !b.Be8 || b.sq < 16 * seg0(nbf)))) return hipErrorInvalidValue;
+  if (xs.nseg < 0 || xs.nseg > kMaxXSeg) return hipErrorInvalidValue;
+  const long bn = 16L * nt, ldk_max = (qpb * bn + 63) / 64 * 64;
+  for (int d = 0; d < xs.nseg; ++d) {
+    const long o = xs.off[d], pad = (xs.len[d] + 15) & ~15;
+    if (o % 16 || xs.len[d] <= 0 || xs.ldk[d] % 64 || xs.ldk[d] > ldk_max || xs.ldk[d] < (long)qpb * xs.len[d] ||
+        o / bn != (o + pad - 1) / bn || o + pad > (long)grid.z * bn || !xs.xd[d] || !xs.xe[d])
       return hipErrorInvalidValue;
-#define NOPT_HALF_I8(NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                  \
-  k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z), 64 * NW_ * QPB_, 0, \
-                                               s>>>(Bd, bplane, Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs)
+  }
+#define NOPT_HALF_I8(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                   \
+  k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z), 64 * NW_ * QPB_, \
+                                                   0, s>>>(b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, \
+                                                           Cd, cplane, Ce, X, ldx, xs)
   // 64-row tiles of one function per block (two blocks per CU), or 32-row
-  // tiles of two functions (two blocks per CU)
+  // tiles of two functions (two blocks per CU); the packed loader wants one
+  // tile row per lane (QPB * BMI = 64)
+  const bool pk = !b.Bd;
   switch (nt * 4 + qpb) {
-    case 2 * 4 + 1: NOPT_HALF_I8(2, 1, 4, 1, 64, 2); break;
-    case 4 * 4 + 1: NOPT_HALF_I8(4, 2, 8, 1, 64, 4); break;
-    case 6 * 4 + 1: NOPT_HALF_I8(6, 2, 8, 1, 64, 4); break;
-    case 2 * 4 + 2: NOPT_HALF_I8(2, 1, 4, 2, 64, 4); break;
-    case 4 * 4 + 2: NOPT_HALF_I8(4, 2, 4, 2, 32, 4); break;
-    case 6 * 4 + 2: NOPT_HALF_I8(6, 2, 4, 2, 32, 4); break;
+    case 2 * 4 + 1: pk ? NOPT_HALF_I8(true, 2, 1, 4, 1, 64, 2) : NOPT_HALF_I8(false, 2, 1, 4, 1, 64, 2); break;
+    case 4 * 4 + 1: pk ? NOPT_HALF_I8(true, 4, 2, 8, 1, 64, 4) : NOPT_HALF_I8(false, 4, 2, 8, 1, 64, 4); break;
+    case 6 * 4 + 1: pk ? NOPT_HALF_I8(true, 6, 2, 8, 1, 64, 4) : NOPT_HALF_I8(false, 6, 2, 8, 1, 64, 4); break;
+    case 2 * 4 + 2: pk ? NOPT_HALF_I8(true, 2, 1, 2, 2, 32, 4) : NOPT_HALF_I8(false, 2, 1, 4, 2, 64, 4); break;
+    case 4 * 4 + 2: pk ? NOPT_HALF_I8(true, 4, 2, 4, 2, 32, 4) : NOPT_HALF_I8(false, 4, 2, 4, 2, 32, 4); break;
+    case 6 * 4 + 2: pk ? NOPT_HALF_I8(true, 6, 2, 4, 2, 32, 4) : NOPT_HALF_I8(false, 6, 2, 4, 2, 32, 4); break;
     default: return hipErrorInvalidValue;
   }
 #undef NOPT_HALF_I8
@@ -1088,11 +1260,26 @@ void coulomb_seg_table(int nbf, int* out) {
     for (int n0 = 0; n0 <= m; n0 += 16) out[i++] = (m << 16) | n0;
 }
 
-hipError_t pack_segments(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, const int* seg_code, int nseg,
-                         int8_t* Bs, long splane, hipStream_t s) {
-  if (naux <= 0 || naux > 65535 || splane < naux * 16L * nseg || ld8 % 16) return hipErrorInvalidValue;
-  k_pack_segments<<<dim3((nseg + 255) / 256, (unsigned)naux), 256, 0, s>>>(Bd, bplane, ld8, nbf, seg_code, nseg, Bs,
-                                                                           splane);
+hipError_t pack_segs_rows(const double* B, long ld, int nbf, long naux, const int* Be, const int* seg_code, int nseg,
+                          int8_t* Bs, long splane, hipStream_t s) {
+  if (naux <= 0 || naux > 65535 || splane < naux * 16L * nseg || ld < nbf) return hipErrorInvalidValue;
+  k_pack_segs<0><<<dim3((nseg + 255) / 256, (unsigned)naux), 256, 0, s>>>(B, ld, 0, nbf, Be, seg_code, 0, nseg,
+                                                                         16L * nseg, Bs, splane);
+  return hipGetLastError();
+}
+
+hipError_t pack_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* Be,
+                           const int* seg_code, int nseg, int8_t* Bs, long splane, hipStream_t s) {
+  if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || splane < naux * 16L * nseg) return hipErrorInvalidValue;
+  const int s0 = (int)seg0(m0), s1 = (int)seg0(m1);
+  k_pack_segs<1><<<dim3((s1 - s0 + 255) / 256, (unsigned)naux), 256, 0, s>>>(A, cp, m0, nbf, Be, seg_code, s0, s1,
+                                                                             16L * nseg, Bs, splane);
+  return hipGetLastError();
+}
+
+hipError_t expo_bytes(const int* Be, int nbf, long ld8, long naux, int8_t* Be8, hipStream_t s) {
+  if (ld8 < nbf) return hipErrorInvalidValue;
+  k_expo8<<<(unsigned)((naux * ld8 + 255) / 256), 256, 0, s>>>(Be, nbf, ld8, naux, Be8);
   return hipGetLastError();
 }
 

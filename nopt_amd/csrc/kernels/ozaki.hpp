@@ -28,12 +28,27 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
 // of 16); its rows get their own exponent xe[d][row] and four digit planes
 // xd[d][p*xplane[d] + row*ldk[d] + col], zero beyond the written columns
 // (ldk[d] a multiple of 64; rows and columns as launch_half_i8 states).
+constexpr int kMaxXSeg = 8;
 struct XSplit {
   int nseg;
-  int off[4], len[4];
-  long ldk[4], xplane[4];
-  int8_t* xd[4];
-  int* xe[4];
+  int off[kMaxXSeg], len[kMaxXSeg];
+  long ldk[kMaxXSeg], xplane[kMaxXSeg];
+  int8_t* xd[kMaxXSeg];
+  int* xe[kMaxXSeg];
+};
+
+// B operand of the half-transform: the full rows as four digit planes
+// (Bd[k*bplane + (Q*nbf + m)*ld8 + n]), or, with Bd null, the packed lower
+// triangle of the Coulomb form below (Bp = (const int*)Bs, its N28 values,
+// sq values per function) with the row exponents also as bytes Be8[Q*ld8 + n].
+// Be holds the row exponents in either case.
+struct HalfB {
+  const int8_t* Bd;
+  long bplane;
+  const int* Bp;
+  long sq;
+  const int8_t* Be8;
+  const int* Be;
 };
 
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
@@ -41,20 +56,22 @@ struct XSplit {
 // column groups of 16*nt columns, nt in {2,4,6}; ld8 a multiple of 64, at
 // most 2048 (nbf <= 2048: the int32 class combine stays exact); the
 // caller's grid.x is ignored).
-// With xs.nseg > 0 (one column group only) the rows are split into digit
-// planes in the epilogue and X is not written. qpb = 2 gives each block the
-// functions 2y and 2y+1, whose rows (y*nbf + m) share one exponent, the
+// With xs.nseg > 0 the rows are split into digit planes in the epilogue and
+// X is not written: segment d covers the columns [off[d], off[d] + len[d])
+// (global column numbers, within one column group). qpb = 2 gives each block
+// the functions 2y and 2y+1, whose rows (y*nbf + m) share one exponent, the
 // function 2y in columns [0, len) and 2y+1 in [len, 2 len) of the segment
 // (ldk >= 2 len); qpb = 1 writes row Q*nbf + m.
-hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const int8_t* Bd, long bplane, const int* Be,
-                          long ld8, int nbf, int naux, const int8_t* Cd, long cplane, const int* Ce, double* X,
-                          long ldx, const XSplit& xs);
+hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB& b, long ld8, int nbf, int naux,
+                          const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx, const XSplit& xs);
 // Coulomb pair over B's lower triangle in packed rows: row m of B_Q as
 // m/16 + 1 segments of 16 columns (zeros beyond column m), the rows one after
 // the other, element e = Q*sq + 16*seg0(m) + n, sq = 16*nseg; each value as
-// the 35-bit integer N of its five digits, ((int*)Bs)[e] = N >> 8 and
-// Bs[4*splane + e] = N & 255 (pack_segments forms them from the full digit
-// rows; seg_code[s] = m << 16 | n0 for each segment; Bs holds 5*splane bytes). The kernels walk 8-column slots (m, n0)
+// its 35-bit integer N35 = 128 N28 + d4, ((int*)Bs)[e] = N28 and
+// ((int8_t*)Bs)[4*splane + e] = d4 (pack_segs_rows forms them from FP64
+// rows, pack_segs_chunk from a packed chunk of rows m0..m1; seg_code[s] =
+// m << 16 | n0 for each segment; Bs holds 5*splane bytes). expo_bytes writes
+// the row exponents as bytes. The kernels walk 8-column slots (m, n0)
 // of the triangle (slot table, m/8 + 1 slots per row): gamma (partials per
 // slot block, then summed) and J partials per Q-split
 // (Jpart[d*jstride + split*nbf*nbf + m*nbf + n], n <= m).
@@ -62,8 +79,11 @@ constexpr int kGammaSPL = 2;  // slots per lane of the gamma kernel
 constexpr int kJSPL = 2;      // slots per thread of the J kernel
 int coulomb_seg_count(int nbf);
 void coulomb_seg_table(int nbf, int* out);
-hipError_t pack_segments(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, const int* seg_code, int nseg,
-                         int8_t* Bs, long splane, hipStream_t s);
+hipError_t pack_segs_rows(const double* B, long ld, int nbf, long naux, const int* Be, const int* seg_code, int nseg,
+                          int8_t* Bs, long splane, hipStream_t s);
+hipError_t pack_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* Be,
+                           const int* seg_code, int nseg, int8_t* Bs, long splane, hipStream_t s);
+hipError_t expo_bytes(const int* Be, int nbf, long ld8, long naux, int8_t* Be8, hipStream_t s);
 int coulomb_slot_count(int nbf);
 void coulomb_slot_table(int nbf, int* out);
 int gamma_slot_blocks(int nslots);

@@ -25,18 +25,22 @@ from . import _lib as L
 
 
 KEEP = {"both": 0, "fp64": 1, "int8": 2}  # NOPT_CTX_DEFAULT, _FP64_ONLY, _INT8_ONLY
+COMPACT = 4  # NOPT_CTX_COMPACT
 
 
 class JKContext:
     """A GPU build context. `keep` selects the forms of B held in HBM: "both"
-    (every mode), "fp64" (FP64/FP32/bf16 modes) or "int8" (integer mode)."""
+    (every mode), "fp64" (FP64/FP32/bf16 modes) or "int8" (integer mode).
+    `compact` keeps the integer form as the packed triangle alone (the
+    half-transform unpacks it; a large system gets it anyway)."""
 
-    def __init__(self, nbf, naux, device=0, keep="both"):
+    def __init__(self, nbf, naux, device=0, keep="both", compact=False):
         lib = L.lib()
         if not hasattr(lib, "nopt_jk_create"):
             raise ImportError("libnoptjk.so was built without the HIP kernels")
         h = ctypes.c_void_p()
-        L.check(lib.nopt_jk_create_ex(int(device), int(nbf), int(naux), KEEP[keep], ctypes.byref(h)))
+        flags = KEEP[keep] | (COMPACT if compact else 0)
+        L.check(lib.nopt_jk_create_ex(int(device), int(nbf), int(naux), flags, ctypes.byref(h)))
         self.keep = keep
         self._h = h
         self.nbf, self.naux = int(nbf), int(naux)

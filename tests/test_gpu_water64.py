@@ -7,7 +7,8 @@ runs integer builds until the DIIS error is below 1e-5 and then hands over to
 an FP64 context for the end-game (src/SCF.cpp:78-223 as iterative
 refinement), and lands within 1e-8 Eh of the FP64 run. On the converged
 density the integer build stays within 1 uEh of the FP64 build. Each context
-keeps one form of B (FP64 rows or integer forms), as a large system must."""
+keeps one form of B (FP64 rows or integer forms), as a large system must;
+the integer-only context fits in 70 GB."""
 import sys
 import time
 
@@ -54,11 +55,19 @@ def test_water64_mixed_rhf_with_fp64_endgame():
     say("FP64 RI-RHF", ref.E, ref.builds, round(time.perf_counter() - t0, 1), "s")
 
     ctx = context("int8")
+    t1 = time.perf_counter()
+    J8, K8 = ctx.build(D, Co, m.nocc, prec="int8")
+    t2 = time.perf_counter()
     J8, K8 = ctx.build(D, Co, m.nocc, prec="int8")
     dE = two_electron_energy(D, J8, K8) - two_electron_energy(D, J64, K64)
     dj, dk = float(np.abs(J8 - J64).max()), float(np.abs(K8 - K64).max())
-    say("int8 vs fp64 at the converged density: dE", dE, "max|dJ|", dj, "max|dK|", dk)
+    say("int8 vs fp64 at the converged density: dE", dE, "max|dJ|", dj, "max|dK|", dk, "; builds",
+        round(t2 - t1, 3), round(time.perf_counter() - t2, 3), "s (host buffers);",
+        round(ctx.device_bytes() / 1e9, 1), "GB on the device")
     assert abs(dE) < 1e-6 and dj < 1e-8 and dk < 5e-7
+    # the integer-only context keeps the packed triangle alone, and the
+    # exchange operand in digit planes per column group: no FP64 X
+    assert ctx.device_bytes() < 70e9
 
     def to_fp64(c):
         c.close()
