@@ -185,7 +185,7 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
     builds launched one by one, with the split counts of each system set by
     the autotuner (the replay with the counts picked at context creation is
     reported beside it)."""
-    from nopt_amd.jk import JKContext, DeviceArray, Graph, synchronize, build_ragged_device
+    from nopt_amd.jk import JKContext, DeviceArray, Graph, RaggedPlan, synchronize, build_ragged_device
     from nopt_amd.workloads import ragged_clusters
     t0 = time.perf_counter()
     systems, sizes = [], []
@@ -265,6 +265,32 @@ def ragged_batch(prec, steps, warmup, count=32, nmax=16, seed=0):
         res[p] = dict(eager_ms=1e3 * eager, graph_ms=1e3 * graph, builds_per_s_graph=2 * count / graph,
                       builds_per_s_eager=2 * count / eager, graph_vs_eager_max_abs_diff=same,
                       graph_ms_default_splits=untuned[p], outputs=ref)
+        if p == "int8":
+            # the work-list plan: every phase of all systems in one launch
+            for d in systems:
+                for o in d["out"]:
+                    o.copy_from(np.zeros(o.shape))
+            with RaggedPlan([(d["ctx"], [(d["dev"][0].ptr, d["dev"][1].ptr, d["ncore"], d["out"][0].ptr,
+                                          d["out"][1].ptr),
+                                         (d["dev"][2].ptr, d["dev"][3].ptr, d["nt"], d["out"][2].ptr,
+                                          d["out"][3].ptr)]) for d in systems]) as plan:
+                plan.launch(stream)
+                synchronize()
+                same_plan = max(float(np.abs(o.to_host() - r).max())
+                                for d, rr in zip(systems, ref) for o, r in zip(d["out"], rr))
+                with Graph.capture(stream) as cap:
+                    plan.launch(stream)
+                g = cap.graph
+                g.launch(stream)
+                synchronize()
+                t1 = time.perf_counter()
+                for _ in range(steps):
+                    g.launch(stream)
+                synchronize()
+                tp = (time.perf_counter() - t1) / steps
+                g.close()
+                res[p].update(plan_graph_ms=1e3 * tp, builds_per_s_plan=2 * count / tp, plan_launches=plan.launches,
+                              plan_vs_eager_max_abs_diff=same_plan)
     # accuracy of the mixed mode against FP64 over the whole batch, as the
     # two-electron energy of each density: sum Dc.(2J - K) for the inactive
     # density (unit occupations, closed shell) and 1/2 sum Dt.J - 1/4 sum Dt.K
@@ -336,17 +362,13 @@ def large_system(n_water, seed, steps, threads):
     def rate(c, D, Co, prec):
         dD, dC = DeviceArray.from_host(D), DeviceArray.from_host(Co)
         dJ, dK = DeviceArray((nbf, nbf)), DeviceArray((nbf, nbf))
-        c.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
-        synchronize()
-        t1 = time.perf_counter()
-        for _ in range(steps):
-            c.build_device(dD.ptr, dC.ptr, nocc, dJ.ptr, dK.ptr, prec)
-        synchronize()
-        dt = (time.perf_counter() - t1) / steps
+        r, _, ph = time_builds(c, dD, dC, nocc, dJ, dK, prec, steps, 1, synchronize, timing=True)
+        timings[f"phase_ms_{prec}"] = {k: round(v, 3) for k, v in ph.items()}
+        timings[f"device_bytes_{prec}_context"] = c.device_bytes()
         out = (dJ.to_host(), dK.to_host())
         for a in (dD, dC, dJ, dK):
             a.free()
-        return 1.0 / dt, out
+        return r, out
 
     ctx = context("fp64")
     t0 = time.perf_counter()
@@ -357,12 +379,13 @@ def large_system(n_water, seed, steps, threads):
     Co = np.ascontiguousarray(ref.C[:, :nocc].T)
     D = Co.T @ Co
     rate64, (J64, K64) = rate(ctx, D, Co, "fp64")
-    log(f"[large] fp64: {rate64:.3f} builds/s")
+    log(f"[large] fp64: {rate64:.3f} builds/s; phases {timings['phase_ms_fp64']}")
     ctx.close()
     ctx = context("int8")
     rate8, (J8, K8) = rate(ctx, D, Co, "int8")
     dE_build = energy_terms(D, J8, K8) - energy_terms(D, J64, K64)
-    log(f"[large] int8: {rate8:.3f} builds/s; on the converged density dE {dE_build:.3e} "
+    log(f"[large] int8: {rate8:.3f} builds/s; phases {timings['phase_ms_int8']}, "
+        f"{timings['device_bytes_int8_context'] / 1e9:.1f} GB on the device; on the converged density dE {dE_build:.3e} "
         f"max|dJ| {np.abs(J8 - J64).max():.3e} max|dK| {np.abs(K8 - K64).max():.3e}")
 
     def to_fp64(c):
@@ -388,8 +411,14 @@ def large_system(n_water, seed, steps, threads):
                                                   "max_abs_dK": float(np.abs(K8 - K64).max())},
                    "host_integrals_s": round(t_ints, 1), **timings,
                    "B_bytes": {"fp64_rows": 8 * naux * nbf * (((nbf + 1 + 15) // 16) * 16),
-                               "digit_planes": 5 * naux * nbf * (((nbf + 63) // 64) * 64)}},
+                               "int8_packed_triangle": 5 * naux * 16 * seg_count(nbf)}},
     }
+
+
+def seg_count(nbf):
+    """16-column segments of the packed lower triangle of one B_Q (row m holds
+    m // 16 + 1 of them)."""
+    return sum(m // 16 + 1 for m in range(nbf))
 
 
 def host_cores():

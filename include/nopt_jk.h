@@ -165,6 +165,29 @@ int nopt_jk_build_ragged_device(int nsys, nopt_jk_ctx* const* ctxs, const int* n
                                 const double* const* dC, const long* nocc, double* const* dJ,
                                 double* const* dK, int prec, void* stream);
 
+/* A ragged batch as a plan (no counterpart in NOPT; its CASSCF builds one
+ * system at a time, src/CAS.cpp:461, 696-707): the integer builds of the
+ * systems, items as in nopt_jk_build_ragged_device, are laid out once, and
+ * each phase of all of them (staging, half-transform, gamma, J, exchange,
+ * outputs) then runs as one launch that walks a work list of (system,
+ * density, tile) items, so that small systems fill the device together.
+ * create grows the contexts' work buffers, records the work lists in device
+ * memory and pins the contexts like a captured graph (no growth or tuning
+ * change until destroy); launch enqueues the phase kernels on `stream` (a
+ * capture on `stream` records them as one graph: they allocate nothing);
+ * the plan reads the inputs and writes the outputs named at create on every
+ * launch. Integer mode only; each system's orbitals must fit the fused
+ * exchange split (at most 8 segments of 96 columns, NOPT_EINVAL otherwise).
+ * Contexts on one device, each at most once. */
+typedef struct nopt_ragged nopt_ragged;
+int nopt_ragged_create(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const double* const* dD,
+                       const double* const* dC, const long* nocc, double* const* dJ, double* const* dK,
+                       nopt_ragged** out);
+int nopt_ragged_launch(nopt_ragged* plan, void* stream);
+/* Kernel launches per nopt_ragged_launch. */
+int nopt_ragged_launches(const nopt_ragged* plan);
+void nopt_ragged_destroy(nopt_ragged* plan);
+
 /* Fock assembly and energy on the device from host buffers:
  * F = H + cj*J - ck*K and E = sum D.*F + sum D.*H (E_1el_calc,
  * src/doCI_matr.cpp:38-47, as used at src/SCF.cpp:134; the caller adds
@@ -188,10 +211,12 @@ int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const dou
                                    int prec, double* dE, double* dG, double* dF, void* stream);
 
 /* Tile and split tuning. The build divides the auxiliary index into Q-splits
- * (partial sums over Q ranges, added in FP64 at the end): params[5] =
+ * (partial sums over Q ranges, added in FP64 at the end): params[6] =
  * {FP64/FP32/bf16 exchange splits, FP64 J splits (0: the fewest the kernel's
  * gamma cache allows, at least 16), integer exchange splits (a multiple of
- * 8), integer gamma splits, integer J splits}. create picks them from the
+ * 8), integer gamma splits, integer J splits, and where the integer Coulomb
+ * pair starts on its side stream: 0 after the half-transform, 1 gamma beside
+ * the half-transform and J after it, 2 both beside it}. create picks them from the
  * device's CU count; nopt_jk_autotune times `reps` device builds of (dD, dC)
  * in mode `prec` on `stream` for each candidate of each split count in turn
  * (two rounds of coordinate descent), keeps the fastest and returns its mean

@@ -69,6 +69,12 @@ GPU_SYMBOLS = [
     ("nopt_jk_build_ragged_device", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, ctypes.POINTER(_vp),
                                                    ctypes.POINTER(_vp), _c_long_p, ctypes.POINTER(_vp),
                                                    ctypes.POINTER(_vp), ctypes.c_int, _vp]),
+    ("nopt_ragged_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), _c_int_p, ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp), _c_long_p, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                          ctypes.POINTER(_vp)]),
+    ("nopt_ragged_launch", ctypes.c_int, [_vp, _vp]),
+    ("nopt_ragged_launches", ctypes.c_int, [_vp]),
+    ("nopt_ragged_destroy", None, [_vp]),
     ("nopt_jk_fock_energy", ctypes.c_int, [_vp, _c_double_p, _c_double_p, _c_double_p, _c_double_p,
                                            ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     ("nopt_jk_energy_gradient", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_int,

@@ -68,3 +68,28 @@ __device__ __forceinline__ double block_sum256(double v, double* sh /* >= 4 */) 
 }
 
 static inline long round_up(long x, long m) { return (x + m - 1) / m * m; }
+
+// Work lists: one launch runs a kernel over the grids of several items (the
+// systems of a ragged batch). Item i owns the blocks [first[i], first[i+1])
+// of the 1-D launch; a kernel body sees its block of the item's own grid as
+// a VB, x fastest.
+struct VB {
+  int x, y, z, gx, gy, gz;
+};
+// the item of block b: the largest i with first[i] <= b (first[0] = 0)
+__device__ __forceinline__ int list_item(const int* __restrict__ first, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+// the launch's own block as a VB
+__device__ __forceinline__ VB grid_vb() {
+  return VB{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.x, (int)gridDim.y, (int)gridDim.z};
+}
+__device__ __forceinline__ VB list_vb(int local, int gx, int gy, int gz) {
+  return VB{local % gx, (local / gx) % gy, local / (gx * gy), gx, gy, gz};
+}

@@ -125,13 +125,35 @@ __global__ void k_active_ints(const double* __restrict__ R, long naux, int nn, d
 // dst[r][c] (ldd) = src[r][c] (lds) for r < rows, c < cols; zero elsewhere
 // in [0, rows_pad) x [0, ldd)
 template <typename T>
-__global__ void k_pad_copy(const double* __restrict__ src, long lds, int rows, int cols,
-                           T* __restrict__ dst, long ldd, int rows_pad) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_pad_copy_body(const VB vb, const double* __restrict__ src, long lds, int rows,
+                                                int cols, T* __restrict__ dst, long ldd, int rows_pad) {
+  const long e = (long)vb.x * blockDim.x + threadIdx.x;
   if (e >= (long)rows_pad * ldd) return;
   const long r = e / ldd, c = e % ldd;
   double v = (r < rows && c < cols) ? src[r * lds + c] : 0.0;
   dst[e] = (T)v;
+}
+template <typename T>
+__global__ void k_pad_copy(const double* __restrict__ src, long lds, int rows, int cols,
+                           T* __restrict__ dst, long ldd, int rows_pad) {
+  k_pad_copy_body<T>(grid_vb(), src, lds, rows, cols, dst, ldd, rows_pad);
+}
+// work-list form (FP64 copies)
+struct ItemPad {
+  const double* src;
+  long lds;
+  int rows;
+  int cols;
+  double* dst;
+  long ldd;
+  int rows_pad;
+  int gx, gy, gz;
+};
+__global__ void k_pad_copy_list(const ItemPad* __restrict__ items, const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemPad& a = items[i];
+  k_pad_copy_body<double>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz), a.src, a.lds, a.rows, a.cols, a.dst,
+                          a.ldd, a.rows_pad);
 }
 
 // --------------------------------------------------- half-transform f64/f32
@@ -513,11 +535,23 @@ __global__ __launch_bounds__(256) void k_syrk(const T* __restrict__ X, long ldx,
 // --------------------------------------------------------------- finalize
 // J = sum_s Jpart[s], K = sum_s Kpart[s] over the lower triangle, written to
 // both triangles of J, K (ld nbf). Either output may be null.
-__global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const double* __restrict__ Kpart,
-                           int nsk, int ntiles, int kt, int nbf, double* __restrict__ J,
-                           double* __restrict__ K) {
+struct ItemFin {
+  const double* Jpart;
+  int nsj;
+  const double* Kpart;
+  int nsk;
+  int ntiles;
+  int kt;
+  int nbf;
+  double* J;
+  double* K;
+  int gx, gy, gz;
+};
+__device__ __forceinline__ void k_finalize_body(const VB vb, const double* __restrict__ Jpart, int nsj,
+                                                const double* __restrict__ Kpart, int nsk, int ntiles, int kt,
+                                                int nbf, double* __restrict__ J, double* __restrict__ K) {
   const int KT = kt;  // exchange tile edge of the partials
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long e = (long)vb.x * blockDim.x + threadIdx.x;
   if (e >= (long)nbf * nbf) return;
   const int m = (int)(e / nbf), n = (int)(e % nbf);
   if (n > m) return;
@@ -537,6 +571,19 @@ __global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const doub
     K[(long)n * nbf + m] = s;
   }
 }
+__global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const double* __restrict__ Kpart,
+                           int nsk, int ntiles, int kt, int nbf, double* __restrict__ J,
+                           double* __restrict__ K) {
+  k_finalize_body(grid_vb(), Jpart, nsj, Kpart, nsk, ntiles, kt, nbf, J, K);
+}
+__global__ void k_finalize_list(const ItemFin* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemFin& a = items[i];
+  k_finalize_body(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.Jpart, a.nsj, a.Kpart, a.nsk, a.ntiles, a.kt, a.nbf, a.J, a.K);
+}
+
 
 // ----------------------------------------------------------- Fock, energy
 __global__ void k_fock_energy(const double* __restrict__ H, const double* __restrict__ D,
@@ -650,6 +697,9 @@ struct nopt_jk_ctx {
   int* segs = nullptr;   // segment table (m << 16 | n0), used to pack Bs
   int* slots = nullptr;  // 8-column slots (m << 16 | n0) walked by the kernels
   int nslots = 0, nsg = 0, nsj8 = 0;
+  // when the integer Coulomb pair starts on the side stream: 0 after the
+  // half-transform; 1 gamma before it (beside it), J after it; 2 both before
+  int cplace = 0;
   double* gslot = nullptr;
   // optional per-phase timing: events for up to kTimedBuilds builds
   bool timing = false;
@@ -1004,8 +1054,35 @@ int carve_work(void* base, size_t cap, const WorkBytes& b, Work* w) {
 // density in one pass over B each; K is one triangular product per density.
 // Phases (for the optional timing): staging, half-transform, gamma, J
 // partials, exchange (K products and K output), J output.
+// A recorded build (the ragged plan): the integer build's launches as
+// work-list items tagged with their phase (launches of one phase do not
+// depend on each other across systems; phases run in order).
+constexpr int kListPad = nopt::kListOzakiKinds, kListFin = nopt::kListOzakiKinds + 1;
+struct RecItem {
+  int phase;
+  nopt::ListRec r;
+};
+struct Rec {
+  std::vector<RecItem> items;
+  template <typename T>
+  void add(int phase, int kind, int blocks, int threads, const T& a) {
+    RecItem x;
+    x.phase = phase;
+    x.r.kind = kind;
+    x.r.cfg = 0;
+    x.r.blocks = blocks;
+    x.r.threads = threads;
+    x.r.args.resize(sizeof(T));
+    std::memcpy(x.r.args.data(), &a, sizeof(T));
+    items.push_back(std::move(x));
+  }
+  void add(int phase, const nopt::ListRec& r) { items.push_back(RecItem{phase, r}); }
+};
+
+// With rec, the integer build is recorded instead of enqueued (no timing, no
+// side stream; the fused exchange split only).
 int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStream_t s, void* work_area = nullptr,
-                  size_t work_area_bytes = 0) {
+                  size_t work_area_bytes = 0, Rec* rec = nullptr) {
   const int nbf = (int)c->nbf, naux = (int)c->naux;
   if (prec < 0 || prec > 4) return set_error(NOPT_EINVAL, "unknown precision mode");
   if (nd < 1 || nd > kMaxBatch) return set_error(NOPT_EINVAL, "batch size must be between 1 and NOPT_MAX_BATCH");
@@ -1079,6 +1156,9 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   long xoff[nopt::kMaxXSeg + 1];  // digit-plane offsets (bytes) of each segment
   xoff[0] = 0;
   for (int k = 0; k < (xfused ? nxs : 0); ++k) xoff[k + 1] = xoff[k] + 4 * rowsq * ldk_of(segl[k].len);
+  if (rec && (!i8 || (anyK && !xfused)))
+    return set_error(NOPT_EINVAL, "a recorded (ragged plan) build needs the integer mode and at most 8 column-group "
+                                  "segments");
   int segs_max = 1;  // most segments of one density (K partial ranges)
   for (int d = 0; d < nd && xfused; ++d) segs_max = std::max(segs_max, sfirst[d + 1] - sfirst[d]);
   // Coulomb splits: nj gamma chunks of one split must fit the LDS cache
@@ -1101,7 +1181,11 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       wb.Xe = sizeof(int) * (xfused ? rowsq * nxs : rows);
     }
   }
-  wb.Kpart = sizeof(double) * (i8 ? (size_t)segs_max * c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
+  // K partials: a range per segment of one density, reused density after
+  // density; a recorded build runs every exchange before any K output, so
+  // there each segment of the batch has its own range
+  const int kranges = rec ? std::max(nxs, 1) : segs_max;
+  wb.Kpart = sizeof(double) * (i8 ? (size_t)kranges * c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile
                                   : (size_t)c->nsk * c->ntiles * KT * KT);
   if (nj) wb.Jpart = sizeof(double) * nj * nsj * nn;
   wb.Dp = sizeof(double) * std::max(nj, 1) * c->dstride;
@@ -1112,7 +1196,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   rc = work_area ? carve_work(work_area, work_area_bytes, wb, &w) : context_work(c, wb, &w);
   if (rc) return rc;
   double* Jpart = (double*)w.Jpart;
-  const bool tm = c->timing && c->timed < kTimedBuilds;
+  const bool tm = !rec && c->timing && c->timed < kTimedBuilds;
   auto mark = [&](int ph) {
     if (tm) (void)hipEventRecord(c->tev[(size_t)c->timed * (kPhases + 1) + ph], s);
   };
@@ -1120,7 +1204,33 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   // run on the side stream, forked after the staging (after the
   // half-transform in the integer mode, or when gamma rides on it) and joined
   // before the build ends, so that they overlap the exchange.
-  const bool overlap = !tm && nj > 0 && anyK;
+  const bool overlap = !rec && !tm && nj > 0 && anyK;
+  const unsigned nelj = (unsigned)((nn + 255) / 256);  // finalize blocks
+  auto pad_rows = [&](int phase, const double* src, long lds, int r, int cols, double* dst, long ldd, int rpad) {
+    const long n = (long)rpad * ldd;
+    if (n <= 0) return;
+    if (rec)
+      rec->add(phase, kListPad, (int)((n + 255) / 256), 256,
+               ItemPad{src, lds, r, cols, dst, ldd, rpad, (int)((n + 255) / 256), 1, 1});
+    else
+      stage_rows<double>(src, lds, r, cols, dst, ldd, rpad, s);
+  };
+  auto split_rows = [&](const double* src, long ld, int cols, long rows_src, int8_t* planes, long ldo, long rows_dst,
+                        long pstride, int* expo) -> hipError_t {
+    if (!rec) return nopt::split_rows_i8(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, s);
+    if (rows_dst <= 0) return hipSuccess;
+    nopt::ListRec r;
+    CHK_(nopt::split_rows_item(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, &r));
+    rec->add(1, r);
+    return hipSuccess;
+  };
+  auto finalize = [&](int phase, const double* Jp, int nsj_, const double* Kp, int nsk_, int ntl, int kt,
+                      double* J, double* K, hipStream_t st) {
+    if (rec)
+      rec->add(phase, kListFin, (int)nelj, 256, ItemFin{Jp, nsj_, Kp, nsk_, ntl, kt, nbf, J, K, (int)nelj, 1, 1});
+    else
+      k_finalize<<<nelj, 256, 0, st>>>(Jp, nsj_, Kp, nsk_, ntl, kt, nbf, J, K);
+  };
   hipStream_t sj = overlap ? c->side : s;
   auto cmark = [&](int ph) {
     if (!overlap) mark(ph);
@@ -1128,6 +1238,13 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   auto coulomb_gamma = [&]() -> hipError_t {
     if (fused) {
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(w.gpart, c->mtiles, naux, w.gamma);
+    } else if (i8 && rec) {
+      nopt::ListRec r[2];
+      CHK_(nopt::gamma_slots_items(nj, c->nsg, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
+                                   c->nslots, c->segs, c->nseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux, r));
+      rec->add(3, r[0]);
+      rec->add(4, r[1]);
+      return hipSuccess;
     } else if (i8) {
       CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
                                     c->nslots, c->segs, c->nseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux));
@@ -1144,6 +1261,13 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     return hipSuccess;
   };
   auto coulomb_j = [&]() -> hipError_t {
+    if (i8 && rec) {
+      nopt::ListRec r;
+      CHK_(nopt::jpart_slots_item(nj, nsj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots, c->nslots,
+                                  c->segs, c->nseg, w.gamma, naux, Jpart, (long)nsj * nn, &r));
+      rec->add(5, r);
+      return hipSuccess;
+    }
     if (i8)
       return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
                                       c->nslots, c->segs, c->nseg, w.gamma, naux, Jpart, (long)nsj * nn);
@@ -1162,8 +1286,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   };
   auto coulomb_out = [&]() -> hipError_t {
     for (int k = 0; k < nj; ++k)
-      k_finalize<<<(unsigned)((nn + 255) / 256), 256, 0, sj>>>(Jpart + (long)k * nsj * nn, nsj, nullptr, 0, 1, KT,
-                                                               nbf, it[jl[k]].J, nullptr);
+      finalize(6, Jpart + (long)k * nsj * nn, nsj, nullptr, 0, 1, KT, it[jl[k]].J, nullptr, sj);
     return hipGetLastError();
   };
   // 0: no J; 1: forked after staging; 2: in line (after the half-transform)
@@ -1171,15 +1294,15 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   mark(0);
   // densities of the J list into slots 0..nj-1 (ld = ldb); orbital blocks
   // into the stacked factor at column offsets off[d] (zero rows between)
-  for (int k = 0; k < nj; ++k) stage_rows<double>(it[jl[k]].D, nbf, nbf, nbf, w.Dp + k * c->dstride, ldb, nbf, s);
+  for (int k = 0; k < nj; ++k) pad_rows(0, it[jl[k]].D, nbf, nbf, nbf, w.Dp + k * c->dstride, ldb, nbf);
   if (anyK) {
     for (int d = 0; d < nd; ++d) {
       const int pad = (int)(off[d + 1] - off[d]);
       if (!pad) continue;
       const int r = (int)it[d].nocc;
       if (i8)
-        HIPCHK(nopt::split_rows_i8(it[d].C, nbf, nbf, r, 4, (int8_t*)w.Cd + off[d] * c->ld8, c->ld8, pad,
-                                   ldx * c->ld8, (int*)w.Ce + off[d], s));
+        HIPCHK(split_rows(it[d].C, nbf, nbf, r, (int8_t*)w.Cd + off[d] * c->ld8, c->ld8, pad, ldx * c->ld8,
+                          (int*)w.Ce + off[d]));
       else if (f64)
         stage_rows<double>(it[d].C, nbf, r, nbf, (double*)w.Cp + off[d] * ldb, ldb, pad, s);
       else
@@ -1187,22 +1310,33 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     }
     const int tail = (int)(ldx - ncol);
     if (i8)
-      HIPCHK(nopt::split_rows_i8(w.Dp, ldb, 0, 0, 4, (int8_t*)w.Cd + ncol * c->ld8, c->ld8, tail, ldx * c->ld8,
-                                 (int*)w.Ce + ncol, s));
+      HIPCHK(split_rows(w.Dp, ldb, 0, 0, (int8_t*)w.Cd + ncol * c->ld8, c->ld8, tail, ldx * c->ld8,
+                        (int*)w.Ce + ncol));
     else if (f64)
       stage_rows<double>(w.Dp, ldb, 0, 0, (double*)w.Cp + ncol * ldb, ldb, tail, s);
     else
       stage_rows<float>(w.Dp, ldb, 0, 0, (float*)w.Cp + ncol * ldb, ldb, tail, s);
   }
   HIPCHK(hipGetLastError());
-  // the integer Coulomb pair starts after the half-transform (it then shares
-  // the device with the exchange; started earlier, whole or gamma alone, it
-  // takes CUs from the half-transform, measured slower)
+  // the integer Coulomb pair starts where cplace says (tuned: beside the
+  // half-transform it takes CUs from it, after it it shares the device with
+  // the exchange)
   if (overlap && !fused && !i8) {
     HIPCHK(hipEventRecord(c->ev_fork, s));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     HIPCHK(coulomb());
     HIPCHK(coulomb_out());
+    coulomb_at = 1;
+  }
+  const bool i8early = overlap && i8 && c->cplace > 0;
+  if (i8early) {
+    HIPCHK(hipEventRecord(c->ev_fork, s));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    HIPCHK(coulomb_gamma());
+    if (c->cplace == 2) {
+      HIPCHK(coulomb_j());
+      HIPCHK(coulomb_out());
+    }
     coulomb_at = 1;
   }
   mark(1);
@@ -1224,8 +1358,15 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
         }
       if (xfused) gh.y = (unsigned)npq;
       const nopt::HalfB hb{c->Bd, rows * c->ld8, reinterpret_cast<const int*>(c->Bs), 16L * c->nseg, c->Be8, c->Be};
-      e = nopt::launch_half_i8(nt, qpb, gh, s, hb, c->ld8, nbf, naux, (const int8_t*)w.Cd, ldx * c->ld8,
-                               (const int*)w.Ce, xfused ? nullptr : (double*)w.X, ldx, xs);
+      if (rec) {
+        nopt::ListRec r;
+        e = nopt::half_i8_item(nt, qpb, gh, hb, c->ld8, nbf, naux, (const int8_t*)w.Cd, ldx * c->ld8,
+                               (const int*)w.Ce, nullptr, ldx, xs, &r);
+        if (e == hipSuccess) rec->add(2, r);
+      } else {
+        e = nopt::launch_half_i8(nt, qpb, gh, s, hb, c->ld8, nbf, naux, (const int8_t*)w.Cd, ldx * c->ld8,
+                                 (const int*)w.Ce, xfused ? nullptr : (double*)w.X, ldx, xs);
+      }
     }
     else if (f64)
       e = launch_half<double>(nt, gh, s, c->B64, ldb, nbf, (const double*)w.Cp, w.Dp, (double*)w.X, ldx,
@@ -1239,10 +1380,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     HIPCHK(e);
   }
   mark(2);
-  if (overlap && (fused || i8)) {
+  if (overlap && (fused || i8) && !(i8early && c->cplace == 2)) {
     HIPCHK(hipEventRecord(c->ev_half, s));
     HIPCHK(hipStreamWaitEvent(c->side, c->ev_half, 0));
-    HIPCHK(coulomb());
+    if (i8early) {
+      HIPCHK(coulomb_j());
+    } else {
+      HIPCHK(coulomb());
+    }
     HIPCHK(coulomb_out());
     coulomb_at = 1;
   }
@@ -1252,7 +1397,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
     if (!it[d].K) continue;
     const int kpad = (int)(off[d + 1] - off[d]);
     if (!kpad) {  // no occupied orbitals: K = 0
-      stage_rows<double>(w.Dp, nbf, 0, 0, it[d].K, nbf, nbf, s);
+      pad_rows(8, w.Dp, nbf, 0, 0, it[d].K, nbf, nbf);
       continue;
     }
     const unsigned nel = (unsigned)((nn + 255) / 256);
@@ -1262,14 +1407,22 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       // the whole density otherwise; each segment fills its own nsk8 ranges
       // of K partials and the finalize sums them all
       const size_t kseg = (size_t)c->nsk8 * c->ntiles8 * nopt::kSyrkI8Tile * nopt::kSyrkI8Tile;
+      const int kfirst = rec ? sfirst[d] : 0;  // the density's first K partial range
       int nsum = 1;
       if (xfused) {
         nsum = sfirst[d + 1] - sfirst[d];
         for (int k = sfirst[d]; k < sfirst[d + 1]; ++k) {
           const long ldk = ldk_of(segl[k].len);
-          HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, (const int8_t*)w.Xd + xoff[k], rowsq * ldk,
-                                      (const int*)w.Xe + (long)k * rowsq, ldk, nbf, (int)npq,
-                                      w.Kpart + (k - sfirst[d]) * kseg));
+          const int8_t* xd = (const int8_t*)w.Xd + xoff[k];
+          const int* xe = (const int*)w.Xe + (long)k * rowsq;
+          double* kp = w.Kpart + (k - sfirst[d] + kfirst) * kseg;
+          if (rec) {
+            nopt::ListRec r;
+            HIPCHK(nopt::syrk_i8_item(c->ntiles8, c->nsk8, xd, rowsq * ldk, xe, ldk, nbf, (int)npq, kp, &r));
+            rec->add(7, r);
+          } else {
+            HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, xd, rowsq * ldk, xe, ldk, nbf, (int)npq, kp));
+          }
         }
       } else {
         const long ldk = ldk_of(it[d].nocc);
@@ -1278,8 +1431,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
         HIPCHK(nopt::launch_syrk_i8(c->ntiles8, c->nsk8, s, (const int8_t*)w.Xd, rows * ldk, (const int*)w.Xe, ldk,
                                     nbf, (int)naux, w.Kpart));
       }
-      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, nsum * c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nbf,
-                                     nullptr, it[d].K);
+      finalize(8, nullptr, 0, w.Kpart + kfirst * kseg, nsum * c->nsk8, c->ntiles8, nopt::kSyrkI8Tile, nullptr,
+               it[d].K, s);
     } else {
       dim3 gk(c->ntiles, c->nsk);
       if (f64)
@@ -1538,6 +1691,7 @@ int nopt_jk_get_tuning(const nopt_jk_ctx* c, int* params) {
   params[2] = c->nsk8;
   params[3] = c->nsg;
   params[4] = c->nsj8;
+  params[5] = c->cplace;
   return NOPT_OK;
 }
 
@@ -1550,13 +1704,15 @@ int nopt_jk_set_tuning(nopt_jk_ctx* c, const int* params) {
     return set_error(NOPT_ESTATE, "nopt_jk_set_tuning: a captured HIP graph refers to this context's buffers");
   const int na = (int)c->naux;
   if (params[0] < 1 || params[0] > na || params[1] < 0 || params[1] > na || params[2] < 8 || params[2] % 8 ||
-      params[2] > std::max(8, na) || params[3] < 1 || params[3] > na || params[4] < 1 || params[4] > na)
+      params[2] > std::max(8, na) || params[3] < 1 || params[3] > na || params[4] < 1 || params[4] > na ||
+      params[5] < 0 || params[5] > 2)
     return set_error(NOPT_EINVAL, "nopt_jk_set_tuning: split counts out of range (see nopt_jk.h)");
   c->nsk = params[0];
   c->nsj = params[1];
   c->nsk8 = params[2];
   c->nsg = params[3];
   c->nsj8 = params[4];
+  c->cplace = params[5];
   return NOPT_OK;
 }
 
@@ -1593,8 +1749,8 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
   const int na = (int)c->naux;
   const bool i8 = prec == NOPT_PREC_INT8;
   // coordinate descent over the split counts the mode uses
-  int* knobs[3];
-  std::vector<int> cand[3];
+  int* knobs[4];
+  std::vector<int> cand[4];
   int nk = 0;
   auto pow2 = [&](int lo, int hi) {
     std::vector<int> v;
@@ -1609,6 +1765,8 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
     cand[nk++] = pow2(1, std::min(64, na));
     knobs[nk] = &c->nsg;
     cand[nk++] = pow2(1, std::min(64, na));
+    knobs[nk] = &c->cplace;
+    cand[nk++] = {0, 1, 2};
   } else {
     knobs[nk] = &c->nsk;
     cand[nk++] = pow2(1, std::min(128, na));
@@ -1985,6 +2143,162 @@ int nopt_jk_build_ragged_device(int nsys, nopt_jk_ctx* const* ctxs, const int* n
     if (rc) return rc;
   }
   return NOPT_OK;
+}
+
+}  // extern "C"
+
+struct nopt_ragged {
+  int device = 0;
+  std::vector<nopt_jk_ctx*> ctxs;
+  void* table = nullptr;  // item arrays and block offsets of every launch
+  struct Launch {
+    int kind, cfg, n, blocks;
+    size_t items, first;  // byte offsets into table
+  };
+  std::vector<Launch> launches;
+};
+
+namespace {
+int launch_ragged_list(const nopt_ragged::Launch& l, const char* base, hipStream_t s) {
+  const void* items = base + l.items;
+  const int* first = reinterpret_cast<const int*>(base + l.first);
+  if (l.kind == kListPad) {
+    k_pad_copy_list<<<l.blocks, 256, 0, s>>>((const ItemPad*)items, first, l.n);
+  } else if (l.kind == kListFin) {
+    k_finalize_list<<<l.blocks, 256, 0, s>>>((const ItemFin*)items, first, l.n);
+  } else {
+    HIPCHK(nopt::launch_list(l.kind, l.cfg, items, first, l.n, l.blocks, s));
+  }
+  HIPCHK(hipGetLastError());
+  return NOPT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int nopt_ragged_create(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const double* const* dD,
+                       const double* const* dC, const long* nocc, double* const* dJ, double* const* dK,
+                       nopt_ragged** out) {
+  nopt::clear_error();
+  if (!out) return set_error(NOPT_EINVAL, "nopt_ragged_create: null output");
+  *out = nullptr;
+  if (nsys < 1 || !ctxs || !nd || !nocc) return set_error(NOPT_EINVAL, "nopt_ragged_create: empty batch");
+  if (t_capturing) return set_error(NOPT_ESTATE, "nopt_ragged_create: not during graph capture");
+  const int dev = ctxs[0] ? ctxs[0]->device : 0;
+  for (int k = 0; k < nsys; ++k) {
+    int rc = check_ctx(ctxs[k], true);
+    if (rc) return rc;
+    if (ctxs[k]->device != dev) return set_error(NOPT_EINVAL, "nopt_ragged_create: all contexts must be on one device");
+    if (nd[k] < 1 || nd[k] > kMaxBatch) return set_error(NOPT_EINVAL, "nopt_ragged_create: 1 <= nd[k] <= NOPT_MAX_BATCH");
+    for (int j = 0; j < k; ++j)
+      if (ctxs[j] == ctxs[k]) return set_error(NOPT_EINVAL, "nopt_ragged_create: a context appears twice");
+  }
+  HIPCHK(hipSetDevice(dev));
+  // record every system's build (buffers grow here, outside any capture)
+  Rec rec;
+  long first = 0;
+  for (int k = 0; k < nsys; ++k) {
+    nopt_jk_ctx* c = ctxs[k];
+    if (c->done_set) HIPCHK(hipEventSynchronize(c->ev_done));
+    BuildItem it[kMaxBatch];
+    for (int d = 0; d < nd[k]; ++d) {
+      const long i = first + d;
+      it[d] = BuildItem{dD ? dD[i] : nullptr, dC ? dC[i] : nullptr, nocc[i], dJ ? dJ[i] : nullptr,
+                        dK ? dK[i] : nullptr};
+    }
+    first += nd[k];
+    const size_t before = rec.items.size();
+    int rc = enqueue_batch(c, nd[k], it, NOPT_PREC_INT8, c->stream, nullptr, 0, &rec);
+    if (rc) return rc;
+    for (size_t i = before; i < rec.items.size(); ++i)
+      if (rec.items[i].r.blocks < 1) return set_error(NOPT_EINVAL, "nopt_ragged_create: empty work item");
+  }
+  // launches: the items of one (phase, kernel, configuration), in phase order
+  std::stable_sort(rec.items.begin(), rec.items.end(), [](const RecItem& a, const RecItem& b) {
+    if (a.phase != b.phase) return a.phase < b.phase;
+    if (a.r.kind != b.r.kind) return a.r.kind < b.r.kind;
+    return a.r.cfg < b.r.cfg;
+  });
+  auto* p = new (std::nothrow) nopt_ragged;
+  if (!p) return set_error(NOPT_ENOMEM, "nopt_ragged_create: host allocation failed");
+  p->device = dev;
+  std::vector<char> host;
+  auto align = [&](size_t a) { host.resize((host.size() + a - 1) / a * a); };
+  for (size_t i = 0; i < rec.items.size();) {
+    size_t j = i;
+    while (j < rec.items.size() && rec.items[j].phase == rec.items[i].phase &&
+           rec.items[j].r.kind == rec.items[i].r.kind && rec.items[j].r.cfg == rec.items[i].r.cfg)
+      ++j;
+    nopt_ragged::Launch l{rec.items[i].r.kind, rec.items[i].r.cfg, (int)(j - i), 0, 0, 0};
+    align(256);
+    l.items = host.size();
+    for (size_t k = i; k < j; ++k) host.insert(host.end(), rec.items[k].r.args.begin(), rec.items[k].r.args.end());
+    align(256);
+    l.first = host.size();
+    std::vector<int> fb(j - i + 1, 0);
+    long tot = 0;
+    for (size_t k = i; k < j; ++k) {
+      fb[k - i] = (int)tot;
+      tot += rec.items[k].r.blocks;
+    }
+    if (tot > 0x7fffffffL) {
+      delete p;
+      return set_error(NOPT_EINVAL, "nopt_ragged_create: too many blocks in one launch");
+    }
+    fb[j - i] = (int)tot;
+    l.blocks = (int)tot;
+    const char* fp = reinterpret_cast<const char*>(fb.data());
+    host.insert(host.end(), fp, fp + sizeof(int) * fb.size());
+    p->launches.push_back(l);
+    i = j;
+  }
+  int rc = dev_alloc(&p->table, std::max<size_t>(host.size(), 256), "ragged plan tables");
+  if (!rc && hipMemcpy(p->table, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+    rc = set_error(NOPT_EHIP, "nopt_ragged_create: copy of the work lists failed");
+  if (rc) {
+    dev_free(p->table);
+    delete p;
+    return rc;
+  }
+  for (int k = 0; k < nsys; ++k) {
+    p->ctxs.push_back(ctxs[k]);
+    ++ctxs[k]->pins;
+  }
+  *out = p;
+  return NOPT_OK;
+}
+
+int nopt_ragged_launch(nopt_ragged* p, void* stream) {
+  nopt::clear_error();
+  if (!p) return set_error(NOPT_EINVAL, "nopt_ragged_launch: null plan");
+  HIPCHK(hipSetDevice(p->device));
+  hipStream_t s = (hipStream_t)stream;
+  for (nopt_jk_ctx* c : p->ctxs) {
+    int rc = order_in(c, s);
+    if (rc) return rc;
+  }
+  for (const auto& l : p->launches) {
+    int rc = launch_ragged_list(l, (const char*)p->table, s);
+    if (rc) return rc;
+  }
+  for (nopt_jk_ctx* c : p->ctxs) {
+    int rc = order_out(c, s);
+    if (rc) return rc;
+  }
+  return NOPT_OK;
+}
+
+int nopt_ragged_launches(const nopt_ragged* p) { return p ? (int)p->launches.size() : 0; }
+
+void nopt_ragged_destroy(nopt_ragged* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  for (nopt_jk_ctx* c : p->ctxs)
+    if (c->done_set) (void)hipEventSynchronize(c->ev_done);
+  dev_free(p->table);
+  for (nopt_jk_ctx* c : p->ctxs)
+    if (--c->pins == 0 && c->zombie) nopt_jk_destroy(c);
+  delete p;
 }
 
 int nopt_jk_build_batch(nopt_jk_ctx* c, int nd, const double* const* D, const double* const* C, const long* nocc,

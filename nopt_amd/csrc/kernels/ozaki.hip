@@ -21,6 +21,7 @@
 // Class sums are combined in FP64 and scaled by the row exponents.
 #include "ozaki.hpp"
 
+#include <cstring>
 #include <type_traits>
 #include <utility>
 
@@ -74,10 +75,10 @@ __device__ __forceinline__ void digits5(double x, int e, int& n28, int& d4) {
 // Splits the rows of a row-major FP64 matrix (rows_src x cols, ld) into ns
 // int8 digit planes planes[k*pstride + row*ldo + col] (rows_dst rows, zero
 // padded) and row exponents. One wavefront per row.
-__global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ src, long ld, int cols,
+__device__ __forceinline__ void k_split_rows_body(const VB vb, const double* __restrict__ src, long ld, int cols,
                                                    long rows_src, int ns, int8_t* __restrict__ planes,
                                                    long ldo, long pstride, int* __restrict__ expo) {
-  const long row = blockIdx.x;
+  const long row = vb.x;
   const int lane = threadIdx.x;
   const bool live = row < rows_src;
   const double* s = src + row * ld;
@@ -96,6 +97,19 @@ __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ sr
     }
   }
 }
+__global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ src, long ld, int cols,
+                                                   long rows_src, int ns, int8_t* __restrict__ planes,
+                                                   long ldo, long pstride, int* __restrict__ expo) {
+  k_split_rows_body(grid_vb(), src, ld, cols, rows_src, ns, planes, ldo, pstride, expo);
+}
+__global__ __launch_bounds__(64) void k_split_rows_list(const ItemSplit* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemSplit& a = items[i];
+  k_split_rows_body(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.src, a.ld, a.cols, a.rows_src, a.ns, a.planes, a.ldo, a.pstride, a.expo);
+}
+
 
 // ------------------------------------------- rows of packed symmetric chunks
 // A chunk of a symmetric tensor in packed lower-triangle form,
@@ -271,11 +285,11 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 // rows, and each loads one column of four stored rows (the lanes of a
 // function read contiguous dwords).
 template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
-__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_half_i8(
+__device__ __forceinline__ void k_half_i8_body(const VB vb, 
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Bp, long sq, const int8_t* __restrict__ Be8,
     const int* __restrict__ Be, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
-    const XSplit xs) {
+    const XSplit& xs) {
   constexpr int BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
   constexpr int WR = NW / WC, IT = BMI / (16 * WR), JT = NT / WC;
   static_assert(NT % WC == 0 && NW % WC == 0 && IT * WR * 16 == BMI, "tile split");
@@ -290,10 +304,10 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int qq = w / NW, w8 = w % NW;  // which Q of the block, wave within it
   const int wr = w8 / WC, wc = w8 % WC;
-  const int m0 = blockIdx.x * BMI;
-  const long Qb = (long)blockIdx.y * QPB;  // first Q of the block
+  const int m0 = vb.x * BMI;
+  const long Qb = (long)vb.y * QPB;  // first Q of the block
   const long Q = Qb + qq;                  // this wave's Q
-  const int cg = blockIdx.z;
+  const int cg = vb.z;
   // B loader: chunk e = t + NTH u is chunk e%4 of row (e/4)%BMI in plane
   // (e%ACQ)/(4 BMI) of the block's Q number e/ACQ (rows beyond nbf and a Q
   // beyond naux load zeros); the row and chunk do not depend on u
@@ -627,7 +641,7 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
   constexpr int kNoExp = -4096;  // exponent of an all-zero row segment
   uint32_t* st = reinterpret_cast<uint32_t*>(&sm[0][0]);
   int(*rexp)[BMI] = reinterpret_cast<int(*)[BMI]>(&sm[1][0]);
-  const long orow = (long)blockIdx.y * nbf;  // first output row of the block
+  const long orow = (long)vb.y * nbf;  // first output row of the block
   for (int d = 0; d < xs.nseg; ++d) {
     if (xs.off[d] / BN != cg) continue;  // a segment of another column group (uniform per block)
     const int o = xs.off[d] - cg * BN, len = xs.len[d], pad = (len + 15) & ~15;  // o: within the group
@@ -720,6 +734,25 @@ __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(W
     __syncthreads();
   }
 }
+template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
+__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_half_i8(
+    const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Bp, long sq, const int8_t* __restrict__ Be8,
+    const int* __restrict__ Be, long ld8, int nbf, int naux,
+    const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
+    const XSplit xs) {
+  k_half_i8_body<NT, WC, NW, QPB, BMI, WPE, PK>(grid_vb(),
+      Bd, bplane, Bp, sq, Be8, Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs);
+}
+template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
+__global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
+void k_half_i8_list(const ItemHalf* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemHalf& a = items[i];
+  k_half_i8_body<NT, WC, NW, QPB, BMI, WPE, PK>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.Bd, a.bplane, a.Bp, a.sq, a.Be8, a.Be, a.ld8, a.nbf, a.naux, a.Cd, a.cplane, a.Ce, a.X, a.ldx, a.xs);
+}
+
 
 // ------------------------------------------------- Coulomb over packed rows
 // The lower triangle of every B_Q is cut into 8-column slots (m, n0), n0 <= m,
@@ -786,12 +819,12 @@ __device__ __forceinline__ double wave_sum_rows(double v) {
 // values held in registers) and reduces across its lanes; the digits of its
 // next Q are loaded while the current one is reduced.
 template <int ND, int SPL, bool SEG>
-__global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+__device__ __forceinline__ void k_gamma_slots_body(const VB vb, const int8_t* __restrict__ Bs, long splane, long sq,
                                                      const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
                                                      const double* __restrict__ Dp, long ldd, long dstride,
                                                      double* __restrict__ gpart) {
-  const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
+  const int sb = vb.x, sp = vb.y, ns = vb.gy;
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   static_assert(!SEG || SPL == 2, "a segment is two slots");
@@ -860,16 +893,34 @@ __global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ 
     }
   }
 }
+template <int ND, int SPL, bool SEG>
+__global__ __launch_bounds__(256) void k_gamma_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+                                                     const int* __restrict__ Be, int nbf, int naux,
+                                                     const int* __restrict__ slots, int nslots,
+                                                     const double* __restrict__ Dp, long ldd, long dstride,
+                                                     double* __restrict__ gpart) {
+  k_gamma_slots_body<ND, SPL, SEG>(grid_vb(), Bs, splane, sq, Be, nbf, naux, slots, nslots, Dp, ldd, dstride, gpart);
+}
+template <int ND, int SPL, bool SEG>
+__global__ __launch_bounds__(256) void k_gamma_slots_list(const ItemGamma* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemGamma& a = items[i];
+  k_gamma_slots_body<ND, SPL, SEG>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.Bs, a.splane, a.sq, a.Be, a.nbf, a.naux, a.slots, a.nslots, a.Dp, a.ldd, a.dstride, a.gpart);
+}
+
 
 // gamma[d][Q] = sum_sb gpart[sb][d][Q]. A block takes 64 consecutive Q of one
 // density with four waves: wave p adds the slot blocks sb = p, p+4, p+8, ...
 // (eight loads in flight at a time), then the four sums are added in the
 // order p = 0, 1, 2, 3, so that the result is reproducible.
-__global__ __launch_bounds__(256) void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, int naux,
+__device__ __forceinline__ void k_gamma_sum_body(const VB vb, const double* __restrict__ gpart, int nsb, int nd,
+    int naux,
                                                    double* __restrict__ gamma, long gstride) {
   __shared__ double part[4][64];
   const int lane = threadIdx.x & 63, p = threadIdx.x >> 6;
-  const int d = blockIdx.y, q = blockIdx.x * 64 + lane;
+  const int d = vb.y, q = vb.x * 64 + lane;
   double s = 0.0;
   if (q < naux) {
     const double* g = gpart + (long)d * naux + q;
@@ -888,18 +939,30 @@ __global__ __launch_bounds__(256) void k_gamma_sum(const double* __restrict__ gp
   __syncthreads();
   if (p == 0 && q < naux) gamma[d * gstride + q] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
+__global__ __launch_bounds__(256) void k_gamma_sum(const double* __restrict__ gpart, int nsb, int nd, int naux,
+                                                   double* __restrict__ gamma, long gstride) {
+  k_gamma_sum_body(grid_vb(), gpart, nsb, nd, naux, gamma, gstride);
+}
+__global__ __launch_bounds__(256) void k_gamma_sum_list(const ItemGammaSum* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemGammaSum& a = items[i];
+  k_gamma_sum_body(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.gpart, a.nsb, a.nd, a.naux, a.gamma, a.gstride);
+}
+
 
 // Jpart[d][split][m][n] = sum_{Q in split} gamma_d[Q] B_Q,mn over the slots:
 // each thread owns SPL slots (8 columns each) and accumulates over the Q of
 // its split in registers, loading the digits of the next Q while the current
 // one is accumulated.
 template <int ND, int SPL, bool SEG>
-__global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+__device__ __forceinline__ void k_jpart_slots_body(const VB vb, const int8_t* __restrict__ Bs, long splane, long sq,
                                                      const int* __restrict__ Be, int nbf, int naux,
                                                      const int* __restrict__ slots, int nslots,
                                                      const double* __restrict__ gamma, long gstride,
                                                      double* __restrict__ Jpart, long jstride) {
-  const int sb = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
+  const int sb = vb.x, sp = vb.y, ns = vb.gy;
   const int q0 = (int)((long)naux * sp / ns), q1 = (int)((long)naux * (sp + 1) / ns);
   int mrow[SPL], ncol[SPL];
   bool live[SPL];
@@ -969,6 +1032,24 @@ __global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ 
     }
   }
 }
+template <int ND, int SPL, bool SEG>
+__global__ __launch_bounds__(256) void k_jpart_slots(const int8_t* __restrict__ Bs, long splane, long sq,
+                                                     const int* __restrict__ Be, int nbf, int naux,
+                                                     const int* __restrict__ slots, int nslots,
+                                                     const double* __restrict__ gamma, long gstride,
+                                                     double* __restrict__ Jpart, long jstride) {
+  k_jpart_slots_body<ND, SPL, SEG>(grid_vb(),
+      Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride);
+}
+template <int ND, int SPL, bool SEG>
+__global__ __launch_bounds__(256) void k_jpart_slots_list(const ItemJ* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemJ& a = items[i];
+  k_jpart_slots_body<ND, SPL, SEG>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.Bs, a.splane, a.sq, a.Be, a.nbf, a.naux, a.slots, a.nslots, a.gamma, a.gstride, a.Jpart, a.jstride);
+}
+
 
 // ------------------------------------------------------------------ exchange
 __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
@@ -994,7 +1075,7 @@ constexpr int KT8 = 64;
 // of one split read the same X rows at about the same time, from that XCD's
 // L2. The grid is 1-D, ntiles * nsplit blocks, nsplit a multiple of 8.
 template <int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_syrk_i8(
+__device__ __forceinline__ void k_syrk_i8_body(const VB vb, 
     const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
     int ntiles, int nsplit, double* __restrict__ Kpart) {
   // NW waves in a (NW/2) x 2 grid: each wave 64/(NW/2) rows (IT row tiles) x 32 columns
@@ -1005,7 +1086,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
   const int lr = lane & 15, lc = lane >> 4;
   const int per_xcd = ntiles * nsplit / 8;
-  const int gidx = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  const int gidx = (vb.x % 8) * per_xcd + vb.x / 8;
   const int tile = gidx % ntiles, sp = gidx / ntiles, ns = nsplit;
   int bi, bj;
   tri_tile(tile, bi, bj);
@@ -1167,6 +1248,22 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2,
         Kt[rr * KT8 + cc] = k64[i][j][v];
       }
 }
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_syrk_i8(
+    const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
+    int ntiles, int nsplit, double* __restrict__ Kpart) {
+  k_syrk_i8_body<NW>(grid_vb(), Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
+}
+template <int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2)))
+void k_syrk_i8_list(const ItemSyrk* __restrict__ items,
+    const int* __restrict__ first, int n) {
+  const int i = list_item(first, n, (int)blockIdx.x);
+  const ItemSyrk& a = items[i];
+  k_syrk_i8_body<NW>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+      a.Xd, a.xplane, a.Xe, a.ldk, a.nbf, a.naux, a.ntiles, a.nsplit, a.Kpart);
+}
+
 
 }  // namespace
 
@@ -1216,8 +1313,9 @@ hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, lo
   return hipGetLastError();
 }
 
-hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB& b, long ld8, int nbf, int naux,
-                          const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx, const XSplit& xs) {
+namespace {
+hipError_t half_i8_check(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, int nbf, int naux, long ldx,
+                         const XSplit& xs) {
   // ld8 <= 2048 keeps 128 A0 + A1 exact in int32 (the epilogue's class combine)
   if (ld8 % 64 != 0 || ld8 > 2048 || (long)grid.z * 16 * nt > ldx || qpb < 1 || qpb > 2)
     return hipErrorInvalidValue;
@@ -1231,6 +1329,26 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB
         o / bn != (o + pad - 1) / bn || o + pad > (long)grid.z * bn || !xs.xd[d] || !xs.xe[d])
       return hipErrorInvalidValue;
   }
+  if (nt != 2 && nt != 4 && nt != 6) return hipErrorInvalidValue;
+  return hipSuccess;
+}
+// rows per block of the launcher's configuration for (nt, qpb, packed)
+int half_i8_bmi(int nt, int qpb, bool pk) { return qpb == 1 || (nt == 2 && !pk) ? 64 : 32; }
+int half_i8_threads(int nt, int qpb, bool pk) {
+  return qpb == 1 ? (nt == 2 ? 256 : 512) : (nt == 2 ? (pk ? 256 : 512) : 512);
+}
+
+template <typename T>
+void put_args(ListRec* r, const T& a) {
+  r->args.resize(sizeof(T));
+  std::memcpy(r->args.data(), &a, sizeof(T));
+}
+}  // namespace
+
+hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB& b, long ld8, int nbf, int naux,
+                          const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx, const XSplit& xs) {
+  const hipError_t ce = half_i8_check(nt, qpb, grid, b, ld8, nbf, naux, ldx, xs);
+  if (ce != hipSuccess) return ce;
 #define NOPT_HALF_I8(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                   \
   k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z), 64 * NW_ * QPB_, \
                                                    0, s>>>(b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, \
@@ -1325,6 +1443,141 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
     case 2: k_jpart_slots<2, 2, true><<<dim3((nseg + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, segs, nseg, gamma, gstride, Jpart, jstride); break;
     case 3: k_jpart_slots<3, 1, false><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
     case 4: k_jpart_slots<4, 1, false><<<dim3((nslots + 255) / 256, nsplit), 256, 0, s>>>(Bs, splane, sq, Be, nbf, naux, slots, nslots, gamma, gstride, Jpart, jstride); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- work lists
+hipError_t split_rows_item(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
+                           long rows_dst, long pstride, int* expo, ListRec* r) {
+  if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo || rows_dst <= 0) return hipErrorInvalidValue;
+  r->kind = kListSplit;
+  r->cfg = 0;
+  r->blocks = (int)rows_dst;
+  r->threads = 64;
+  put_args(r, ItemSplit{src, ld, cols, rows_src, ns, planes, ldo, pstride, expo, (int)rows_dst, 1, 1});
+  return hipSuccess;
+}
+
+hipError_t half_i8_item(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, int nbf, int naux, const int8_t* Cd,
+                        long cplane, const int* Ce, double* X, long ldx, const XSplit& xs, ListRec* r) {
+  const hipError_t ce = half_i8_check(nt, qpb, grid, b, ld8, nbf, naux, ldx, xs);
+  if (ce != hipSuccess) return ce;
+  const bool pk = !b.Bd;
+  const int bmi = half_i8_bmi(nt, qpb, pk);
+  const int gx = (nbf + bmi - 1) / bmi;
+  r->kind = kListHalf;
+  r->cfg = nt * 4 + qpb + (pk ? 64 : 0);
+  r->blocks = gx * (int)grid.y * (int)grid.z;
+  r->threads = half_i8_threads(nt, qpb, pk);
+  put_args(r, ItemHalf{b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs, gx,
+                       (int)grid.y, (int)grid.z});
+  return hipSuccess;
+}
+
+hipError_t gamma_slots_items(int nd, int nsplit, const int8_t* Bs, long splane, long sq, const int* Be, int nbf,
+                             int naux, const int* slots, int nslots, const int* segs, int nseg, const double* Dp,
+                             long ldd, long dstride, double* gpart, double* gamma, long gstride, ListRec* r) {
+  if (nd < 1 || nd > 4 || nsplit < 1) return hipErrorInvalidValue;
+  const bool seg = nd <= 2;
+  const int nsb = seg ? (nseg + 63) / 64 : gamma_slot_blocks(nslots);
+  r[0].kind = kListGamma;
+  r[0].cfg = nd;
+  r[0].blocks = nsb * nsplit;
+  r[0].threads = 256;
+  put_args(&r[0], ItemGamma{Bs, splane, sq, Be, nbf, naux, seg ? segs : slots, seg ? nseg : nslots, Dp, ldd, dstride,
+                            gpart, nsb, nsplit, 1});
+  r[1].kind = kListGammaSum;
+  r[1].cfg = 0;
+  r[1].blocks = (naux + 63) / 64 * nd;
+  r[1].threads = 256;
+  put_args(&r[1], ItemGammaSum{gpart, nsb, nd, naux, gamma, gstride, (naux + 63) / 64, nd, 1});
+  return hipSuccess;
+}
+
+hipError_t jpart_slots_item(int nd, int nsplit, const int8_t* Bs, long splane, long sq, const int* Be, int nbf,
+                            int naux, const int* slots, int nslots, const int* segs, int nseg, const double* gamma,
+                            long gstride, double* Jpart, long jstride, ListRec* r) {
+  if (nd < 1 || nd > 4 || nsplit < 1) return hipErrorInvalidValue;
+  const bool seg = nd <= 2;
+  const int gx = seg ? (nseg + 255) / 256 : (nslots + 255) / 256;
+  r->kind = kListJ;
+  r->cfg = nd;
+  r->blocks = gx * nsplit;
+  r->threads = 256;
+  put_args(r, ItemJ{Bs, splane, sq, Be, nbf, naux, seg ? segs : slots, seg ? nseg : nslots, gamma, gstride, Jpart,
+                    jstride, gx, nsplit, 1});
+  return hipSuccess;
+}
+
+hipError_t syrk_i8_item(int ntiles, int nsplit, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
+                        int naux, double* Kpart, ListRec* r) {
+  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
+  r->kind = kListSyrk;
+  r->cfg = 4;
+  r->blocks = ntiles * nsplit;  // a multiple of 8: the XCD numbering of k_syrk_i8 holds within the item
+  r->threads = 256;
+  put_args(r, ItemSyrk{Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart, ntiles * nsplit, 1, 1});
+  return hipSuccess;
+}
+
+hipError_t launch_list(int kind, int cfg, const void* items, const int* first, int n, int blocks, hipStream_t s) {
+  if (n < 1 || blocks < 1) return hipErrorInvalidValue;
+  switch (kind) {
+    case kListSplit:
+      k_split_rows_list<<<blocks, 64, 0, s>>>((const ItemSplit*)items, first, n);
+      break;
+    case kListHalf: {
+      const ItemHalf* it = (const ItemHalf*)items;
+#define NOPT_HALF_L(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_) \
+  k_half_i8_list<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<blocks, 64 * NW_ * QPB_, 0, s>>>(it, first, n)
+      switch (cfg) {
+        case 2 * 4 + 1: NOPT_HALF_L(false, 2, 1, 4, 1, 64, 2); break;
+        case 4 * 4 + 1: NOPT_HALF_L(false, 4, 2, 8, 1, 64, 4); break;
+        case 6 * 4 + 1: NOPT_HALF_L(false, 6, 2, 8, 1, 64, 4); break;
+        case 2 * 4 + 2: NOPT_HALF_L(false, 2, 1, 4, 2, 64, 4); break;
+        case 4 * 4 + 2: NOPT_HALF_L(false, 4, 2, 4, 2, 32, 4); break;
+        case 6 * 4 + 2: NOPT_HALF_L(false, 6, 2, 4, 2, 32, 4); break;
+        case 64 + 2 * 4 + 1: NOPT_HALF_L(true, 2, 1, 4, 1, 64, 2); break;
+        case 64 + 4 * 4 + 1: NOPT_HALF_L(true, 4, 2, 8, 1, 64, 4); break;
+        case 64 + 6 * 4 + 1: NOPT_HALF_L(true, 6, 2, 8, 1, 64, 4); break;
+        case 64 + 2 * 4 + 2: NOPT_HALF_L(true, 2, 1, 2, 2, 32, 4); break;
+        case 64 + 4 * 4 + 2: NOPT_HALF_L(true, 4, 2, 4, 2, 32, 4); break;
+        case 64 + 6 * 4 + 2: NOPT_HALF_L(true, 6, 2, 4, 2, 32, 4); break;
+        default: return hipErrorInvalidValue;
+      }
+#undef NOPT_HALF_L
+      break;
+    }
+    case kListGamma: {
+      const ItemGamma* it = (const ItemGamma*)items;
+      switch (cfg) {
+        case 1: k_gamma_slots_list<1, 2, true><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 2: k_gamma_slots_list<2, 2, true><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 3: k_gamma_slots_list<3, kGammaSPL, false><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 4: k_gamma_slots_list<4, kGammaSPL, false><<<blocks, 256, 0, s>>>(it, first, n); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    case kListGammaSum:
+      k_gamma_sum_list<<<blocks, 256, 0, s>>>((const ItemGammaSum*)items, first, n);
+      break;
+    case kListJ: {
+      const ItemJ* it = (const ItemJ*)items;
+      switch (cfg) {
+        case 1: k_jpart_slots_list<1, 2, true><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 2: k_jpart_slots_list<2, 2, true><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 3: k_jpart_slots_list<3, 1, false><<<blocks, 256, 0, s>>>(it, first, n); break;
+        case 4: k_jpart_slots_list<4, 1, false><<<blocks, 256, 0, s>>>(it, first, n); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    case kListSyrk:
+      k_syrk_i8_list<4><<<blocks, 256, 0, s>>>((const ItemSyrk*)items, first, n);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -2,6 +2,8 @@
 #pragma once
 #include "common.hpp"
 
+#include <vector>
+
 namespace nopt {
 // Rows of src (rows_src x cols, ld) -> ns digit planes
 // planes[k*pstride + row*ldo + col] (rows_dst >= rows_src rows, extra rows and
@@ -103,4 +105,112 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
 constexpr int kSyrkI8Tile = 64;
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart);
+
+// Work lists (common.hpp): the kernels above can run the grids of several
+// systems in one launch. A ListRec is one item: the kernel (kind and template
+// configuration), its arguments and its number of blocks. The *_item
+// functions check their arguments as the launchers above do and fill recs
+// instead of launching; launch_list runs n items of one kind and
+// configuration, given their arguments (an array of the kind's struct) and
+// first[i], the first block of item i (first[n] = blocks).
+struct ListRec {
+  int kind = 0, cfg = 0, blocks = 0, threads = 0;
+  std::vector<unsigned char> args;
+};
+enum ListKind { kListSplit = 0, kListHalf, kListGamma, kListGammaSum, kListJ, kListSyrk, kListOzakiKinds };
+struct ItemSplit {
+  const double* src;
+  long ld;
+  int cols;
+  long rows_src;
+  int ns;
+  int8_t* planes;
+  long ldo;
+  long pstride;
+  int* expo;
+  int gx, gy, gz;
+};
+struct ItemHalf {
+  const int8_t* Bd;
+  long bplane;
+  const int* Bp;
+  long sq;
+  const int8_t* Be8;
+  const int* Be;
+  long ld8;
+  int nbf;
+  int naux;
+  const int8_t* Cd;
+  long cplane;
+  const int* Ce;
+  double* X;
+  long ldx;
+  XSplit xs;
+  int gx, gy, gz;
+};
+struct ItemGamma {
+  const int8_t* Bs;
+  long splane;
+  long sq;
+  const int* Be;
+  int nbf;
+  int naux;
+  const int* slots;
+  int nslots;
+  const double* Dp;
+  long ldd;
+  long dstride;
+  double* gpart;
+  int gx, gy, gz;
+};
+struct ItemGammaSum {
+  const double* gpart;
+  int nsb;
+  int nd;
+  int naux;
+  double* gamma;
+  long gstride;
+  int gx, gy, gz;
+};
+struct ItemJ {
+  const int8_t* Bs;
+  long splane;
+  long sq;
+  const int* Be;
+  int nbf;
+  int naux;
+  const int* slots;
+  int nslots;
+  const double* gamma;
+  long gstride;
+  double* Jpart;
+  long jstride;
+  int gx, gy, gz;
+};
+struct ItemSyrk {
+  const int8_t* Xd;
+  long xplane;
+  const int* Xe;
+  long ldk;
+  int nbf;
+  int naux;
+  int ntiles;
+  int nsplit;
+  double* Kpart;
+  int gx, gy, gz;
+};
+hipError_t split_rows_item(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
+                           long rows_dst, long pstride, int* expo, ListRec* r);
+hipError_t half_i8_item(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, int nbf, int naux, const int8_t* Cd,
+                        long cplane, const int* Ce, double* X, long ldx, const XSplit& xs, ListRec* r);
+// two recs: the gamma partials and their sum
+hipError_t gamma_slots_items(int nd, int nsplit, const int8_t* Bs, long splane, long sq, const int* Be, int nbf,
+                             int naux, const int* slots, int nslots, const int* segs, int nseg, const double* Dp,
+                             long ldd, long dstride, double* gpart, double* gamma, long gstride, ListRec* r);
+hipError_t jpart_slots_item(int nd, int nsplit, const int8_t* Bs, long splane, long sq, const int* Be, int nbf,
+                            int naux, const int* slots, int nslots, const int* segs, int nseg, const double* gamma,
+                            long gstride, double* Jpart, long jstride, ListRec* r);
+hipError_t syrk_i8_item(int ntiles, int nsplit, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
+                        int naux, double* Kpart, ListRec* r);
+hipError_t launch_list(int kind, int cfg, const void* items, const int* first, int n, int blocks, hipStream_t s);
 }  // namespace nopt

@@ -176,11 +176,12 @@ class JKContext:
                                                    (ctypes.c_long * nd)(*[int(it[2]) for it in items]),
                                                    col(3), col(4), p, stream or None))
 
-    TUNING_KEYS = ("nsk", "nsj", "nsk8", "nsg", "nsj8")
+    TUNING_KEYS = ("nsk", "nsj", "nsk8", "nsg", "nsj8", "cplace")
 
     def tuning(self):
-        """Q-split counts of the build (include/nopt_jk.h, nopt_jk_get_tuning)."""
-        a = np.zeros(5, dtype=np.int32)
+        """Q-split counts and the integer Coulomb placement of the build
+        (include/nopt_jk.h, nopt_jk_get_tuning)."""
+        a = np.zeros(len(self.TUNING_KEYS), dtype=np.int32)
         L.check(L.lib().nopt_jk_get_tuning(self._h, a.ctypes.data_as(L._c_int_p)))
         return dict(zip(self.TUNING_KEYS, (int(x) for x in a)))
 
@@ -342,6 +343,46 @@ def build_ragged_device(systems, prec="fp64", stream=0):
         nsys, (ctypes.c_void_p * nsys)(*[c.handle.value for c, _ in systems]),
         (ctypes.c_int * nsys)(*[len(items) for _, items in systems]), col(0), col(1),
         (ctypes.c_long * n)(*[int(it[2]) for it in flat]), col(3), col(4), p, stream or None))
+
+
+class RaggedPlan:
+    """A ragged batch of integer builds as work-list launches (include/nopt_jk.h,
+    nopt_ragged_create): `systems` as for build_ragged_device. The contexts stay
+    pinned until close()."""
+
+    def __init__(self, systems):
+        nsys = len(systems)
+        flat = [it for _, items in systems for it in items]
+        n = len(flat)
+        col = lambda i: (ctypes.c_void_p * n)(*[(it[i] or None) for it in flat])
+        h = ctypes.c_void_p()
+        L.check(L.lib().nopt_ragged_create(
+            nsys, (ctypes.c_void_p * nsys)(*[c.handle.value for c, _ in systems]),
+            (ctypes.c_int * nsys)(*[len(items) for _, items in systems]), col(0), col(1),
+            (ctypes.c_long * n)(*[int(it[2]) for it in flat]), col(3), col(4), ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def launches(self):
+        return int(L.lib().nopt_ragged_launches(self._h))
+
+    def launch(self, stream=0):
+        L.check(L.lib().nopt_ragged_launch(self._h, stream or None))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().nopt_ragged_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def synchronize(device=0):
