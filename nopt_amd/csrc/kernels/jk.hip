@@ -2152,10 +2152,14 @@ struct nopt_ragged {
   std::vector<nopt_jk_ctx*> ctxs;
   void* table = nullptr;  // item arrays and block offsets of every launch
   struct Launch {
-    int kind, cfg, n, blocks;
+    int phase, kind, cfg, n, blocks;
     size_t items, first;  // byte offsets into table
   };
   std::vector<Launch> launches;
+  // the Coulomb phases (gamma, J, J output) run on a second stream beside
+  // the exchange, forked after the half-transform and joined at the end
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -2229,7 +2233,7 @@ int nopt_ragged_create(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const 
     while (j < rec.items.size() && rec.items[j].phase == rec.items[i].phase &&
            rec.items[j].r.kind == rec.items[i].r.kind && rec.items[j].r.cfg == rec.items[i].r.cfg)
       ++j;
-    nopt_ragged::Launch l{rec.items[i].r.kind, rec.items[i].r.cfg, (int)(j - i), 0, 0, 0};
+    nopt_ragged::Launch l{rec.items[i].phase, rec.items[i].r.kind, rec.items[i].r.cfg, (int)(j - i), 0, 0, 0};
     align(256);
     l.items = host.size();
     for (size_t k = i; k < j; ++k) host.insert(host.end(), rec.items[k].r.args.begin(), rec.items[k].r.args.end());
@@ -2255,8 +2259,15 @@ int nopt_ragged_create(int nsys, nopt_jk_ctx* const* ctxs, const int* nd, const 
   int rc = dev_alloc(&p->table, std::max<size_t>(host.size(), 256), "ragged plan tables");
   if (!rc && hipMemcpy(p->table, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
     rc = set_error(NOPT_EHIP, "nopt_ragged_create: copy of the work lists failed");
+  if (!rc && (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+              hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+              hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess))
+    rc = set_error(NOPT_EHIP, "nopt_ragged_create: stream or event creation failed");
   if (rc) {
     dev_free(p->table);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     delete p;
     return rc;
   }
@@ -2277,9 +2288,23 @@ int nopt_ragged_launch(nopt_ragged* p, void* stream) {
     int rc = order_in(c, s);
     if (rc) return rc;
   }
+  // phases 0-2 (staging, C digits, half-transform) and 7-8 (exchange, K
+  // output) on the caller's stream, 3-6 (gamma, J, J output) on the side
+  // stream from the end of the half-transform on
+  bool forked = false;
   for (const auto& l : p->launches) {
-    int rc = launch_ragged_list(l, (const char*)p->table, s);
+    const bool coulomb = l.phase >= 3 && l.phase <= 6;
+    if (coulomb && !forked) {
+      HIPCHK(hipEventRecord(p->ev_fork, s));
+      HIPCHK(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+      forked = true;
+    }
+    int rc = launch_ragged_list(l, (const char*)p->table, coulomb ? p->side : s);
     if (rc) return rc;
+  }
+  if (forked) {
+    HIPCHK(hipEventRecord(p->ev_join, p->side));
+    HIPCHK(hipStreamWaitEvent(s, p->ev_join, 0));
   }
   for (nopt_jk_ctx* c : p->ctxs) {
     int rc = order_out(c, s);
@@ -2295,7 +2320,11 @@ void nopt_ragged_destroy(nopt_ragged* p) {
   (void)hipSetDevice(p->device);
   for (nopt_jk_ctx* c : p->ctxs)
     if (c->done_set) (void)hipEventSynchronize(c->ev_done);
+  if (p->side) (void)hipStreamSynchronize(p->side);
   dev_free(p->table);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   for (nopt_jk_ctx* c : p->ctxs)
     if (--c->pins == 0 && c->zombie) nopt_jk_destroy(c);
   delete p;
