@@ -1332,10 +1332,38 @@ hipError_t half_i8_check(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, i
   if (nt != 2 && nt != 4 && nt != 6) return hipErrorInvalidValue;
   return hipSuccess;
 }
-// rows per block of the launcher's configuration for (nt, qpb, packed)
-int half_i8_bmi(int nt, int qpb, bool pk) { return qpb == 1 || (nt == 2 && !pk) ? 64 : 32; }
+// The half-transform's configurations, one per (packed, nt, qpb):
+// X(PK, NT, WC, NW, QPB, BMI, WPE). 64-row tiles of one function per block
+// (two blocks per CU), or 32-row tiles of two functions (two blocks per CU);
+// the packed loader wants one tile row per lane (QPB * BMI = 64).
+#define NOPT_HALF_CONFIGS(X)   \
+  X(false, 2, 1, 4, 1, 64, 2)  \
+  X(false, 4, 2, 8, 1, 64, 4)  \
+  X(false, 6, 2, 8, 1, 64, 4)  \
+  X(false, 2, 1, 4, 2, 64, 4)  \
+  X(false, 4, 2, 4, 2, 32, 4)  \
+  X(false, 6, 2, 4, 2, 32, 4)  \
+  X(true, 2, 1, 4, 1, 64, 2)   \
+  X(true, 4, 2, 8, 1, 64, 4)   \
+  X(true, 6, 2, 8, 1, 64, 4)   \
+  X(true, 2, 1, 2, 2, 32, 4)   \
+  X(true, 4, 2, 4, 2, 32, 4)   \
+  X(true, 6, 2, 4, 2, 32, 4)
+constexpr int half_cfg(bool pk, int nt, int qpb) { return (pk ? 64 : 0) + nt * 4 + qpb; }
+// rows per block and threads of a configuration (0: none)
+int half_i8_bmi(int nt, int qpb, bool pk) {
+#define NOPT_HALF_BMI(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_) \
+  if (half_cfg(pk, nt, qpb) == half_cfg(PK_, NT_, QPB_)) return BMI_;
+  NOPT_HALF_CONFIGS(NOPT_HALF_BMI)
+#undef NOPT_HALF_BMI
+  return 0;
+}
 int half_i8_threads(int nt, int qpb, bool pk) {
-  return qpb == 1 ? (nt == 2 ? 256 : 512) : (nt == 2 ? (pk ? 256 : 512) : 512);
+#define NOPT_HALF_NTH(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_) \
+  if (half_cfg(pk, nt, qpb) == half_cfg(PK_, NT_, QPB_)) return 64 * NW_ * QPB_;
+  NOPT_HALF_CONFIGS(NOPT_HALF_NTH)
+#undef NOPT_HALF_NTH
+  return 0;
 }
 
 template <typename T>
@@ -1349,21 +1377,15 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB
                           const int8_t* Cd, long cplane, const int* Ce, double* X, long ldx, const XSplit& xs) {
   const hipError_t ce = half_i8_check(nt, qpb, grid, b, ld8, nbf, naux, ldx, xs);
   if (ce != hipSuccess) return ce;
-#define NOPT_HALF_I8(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                   \
-  k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z), 64 * NW_ * QPB_, \
-                                                   0, s>>>(b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, \
-                                                           Cd, cplane, Ce, X, ldx, xs)
-  // 64-row tiles of one function per block (two blocks per CU), or 32-row
-  // tiles of two functions (two blocks per CU); the packed loader wants one
-  // tile row per lane (QPB * BMI = 64)
-  const bool pk = !b.Bd;
-  switch (nt * 4 + qpb) {
-    case 2 * 4 + 1: pk ? NOPT_HALF_I8(true, 2, 1, 4, 1, 64, 2) : NOPT_HALF_I8(false, 2, 1, 4, 1, 64, 2); break;
-    case 4 * 4 + 1: pk ? NOPT_HALF_I8(true, 4, 2, 8, 1, 64, 4) : NOPT_HALF_I8(false, 4, 2, 8, 1, 64, 4); break;
-    case 6 * 4 + 1: pk ? NOPT_HALF_I8(true, 6, 2, 8, 1, 64, 4) : NOPT_HALF_I8(false, 6, 2, 8, 1, 64, 4); break;
-    case 2 * 4 + 2: pk ? NOPT_HALF_I8(true, 2, 1, 2, 2, 32, 4) : NOPT_HALF_I8(false, 2, 1, 4, 2, 64, 4); break;
-    case 4 * 4 + 2: pk ? NOPT_HALF_I8(true, 4, 2, 4, 2, 32, 4) : NOPT_HALF_I8(false, 4, 2, 4, 2, 32, 4); break;
-    case 6 * 4 + 2: pk ? NOPT_HALF_I8(true, 6, 2, 4, 2, 32, 4) : NOPT_HALF_I8(false, 6, 2, 4, 2, 32, 4); break;
+#define NOPT_HALF_I8(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_)                                                  \
+  case half_cfg(PK_, NT_, QPB_):                                                                          \
+    k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z),       \
+                                                     64 * NW_ * QPB_, 0, s>>>(b.Bd, b.bplane, b.Bp, b.sq,  \
+                                                                              b.Be8, b.Be, ld8, nbf, naux, \
+                                                                              Cd, cplane, Ce, X, ldx, xs); \
+    break;
+  switch (half_cfg(!b.Bd, nt, qpb)) {
+    NOPT_HALF_CONFIGS(NOPT_HALF_I8)
     default: return hipErrorInvalidValue;
   }
 #undef NOPT_HALF_I8
@@ -1468,7 +1490,7 @@ hipError_t half_i8_item(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, in
   const int bmi = half_i8_bmi(nt, qpb, pk);
   const int gx = (nbf + bmi - 1) / bmi;
   r->kind = kListHalf;
-  r->cfg = nt * 4 + qpb + (pk ? 64 : 0);
+  r->cfg = half_cfg(pk, nt, qpb);
   r->blocks = gx * (int)grid.y * (int)grid.z;
   r->threads = half_i8_threads(nt, qpb, pk);
   put_args(r, ItemHalf{b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs, gx,
@@ -1530,21 +1552,12 @@ hipError_t launch_list(int kind, int cfg, const void* items, const int* first, i
       break;
     case kListHalf: {
       const ItemHalf* it = (const ItemHalf*)items;
-#define NOPT_HALF_L(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_) \
-  k_half_i8_list<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<blocks, 64 * NW_ * QPB_, 0, s>>>(it, first, n)
+#define NOPT_HALF_L(PK_, NT_, WC_, NW_, QPB_, BMI_, WPE_)                                       \
+  case half_cfg(PK_, NT_, QPB_):                                                               \
+    k_half_i8_list<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<blocks, 64 * NW_ * QPB_, 0, s>>>(it, first, n); \
+    break;
       switch (cfg) {
-        case 2 * 4 + 1: NOPT_HALF_L(false, 2, 1, 4, 1, 64, 2); break;
-        case 4 * 4 + 1: NOPT_HALF_L(false, 4, 2, 8, 1, 64, 4); break;
-        case 6 * 4 + 1: NOPT_HALF_L(false, 6, 2, 8, 1, 64, 4); break;
-        case 2 * 4 + 2: NOPT_HALF_L(false, 2, 1, 4, 2, 64, 4); break;
-        case 4 * 4 + 2: NOPT_HALF_L(false, 4, 2, 4, 2, 32, 4); break;
-        case 6 * 4 + 2: NOPT_HALF_L(false, 6, 2, 4, 2, 32, 4); break;
-        case 64 + 2 * 4 + 1: NOPT_HALF_L(true, 2, 1, 4, 1, 64, 2); break;
-        case 64 + 4 * 4 + 1: NOPT_HALF_L(true, 4, 2, 8, 1, 64, 4); break;
-        case 64 + 6 * 4 + 1: NOPT_HALF_L(true, 6, 2, 8, 1, 64, 4); break;
-        case 64 + 2 * 4 + 2: NOPT_HALF_L(true, 2, 1, 2, 2, 32, 4); break;
-        case 64 + 4 * 4 + 2: NOPT_HALF_L(true, 4, 2, 4, 2, 32, 4); break;
-        case 64 + 6 * 4 + 2: NOPT_HALF_L(true, 6, 2, 4, 2, 32, 4); break;
+        NOPT_HALF_CONFIGS(NOPT_HALF_L)
         default: return hipErrorInvalidValue;
       }
 #undef NOPT_HALF_L
