@@ -137,7 +137,7 @@ def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
               "bf16x3": PEAK["bf16_TF"] / 3}[prec]
         return mfma(f_k, pk)
     if phase == "gamma":
-        if prec == "int8":  # five digit planes of the lower triangle
+        if prec == "int8":  # lower triangle: int32 high plane + signed-byte low digit
             return hbm(5.0 * naux * npair)
         return None  # partials ride on the half-transform
     if phase == "coulomb_j":
