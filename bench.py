@@ -382,6 +382,8 @@ def large_system(n_water, seed, steps, threads):
     log(f"[large] fp64: {rate64:.3f} builds/s; phases {timings['phase_ms_fp64']}")
     ctx.close()
     ctx = context("int8")
+    timings["zero_blocks"] = ctx.zero_blocks()
+    log(f"[large] shares read after the exact-zero blocks are left out: {timings['zero_blocks']}")
     rate8, (J8, K8) = rate(ctx, D, Co, "int8")
     dE_build = energy_terms(D, J8, K8) - energy_terms(D, J64, K64)
     log(f"[large] int8: {rate8:.3f} builds/s; phases {timings['phase_ms_int8']}, "
@@ -577,6 +579,8 @@ def main():
     t_B = time.perf_counter() - t0
     del eri3
     log(f"[bench] B formed on the device (Cholesky + triangular solve) in {t_B:.2f} s")
+    zero_blocks = ctx.zero_blocks()
+    log(f"[bench] shares of the integer forms read after the exact-zero blocks are left out: {zero_blocks}")
     t0 = time.perf_counter()
     res = rhf(mol, ctx=ctx, H=T + V, S=S, prec="fp64", e_conv=1e-10, d_conv=1e-8)
     t_scf = time.perf_counter() - t0
@@ -692,6 +696,7 @@ def main():
             "fp64_builds_per_s": round(rate64, 2),
             "speedup_vs_fp64": round(rate / rate64, 3),
             "host_path_builds_per_s": round(host_rate, 2),
+            "int8_shares_read": {k: round(v, 4) for k, v in zero_blocks.items()},
             "modes": {p: {"builds_per_s": round(runs[p][0], 2), "dE_vs_fp64_Eh": acc[p]["dE"],
                           "max_abs_dK": acc[p]["max_abs_dK"],
                           "phase_ms": {k: round(v, 4) for k, v in runs[p][2].items()},

@@ -96,6 +96,15 @@ int nopt_jk_get_B(nopt_jk_ctx* ctx, double* B_packed);
 /* Device bytes held by the context. */
 long nopt_jk_device_bytes(const nopt_jk_ctx* ctx);
 
+/* The share of B's integer forms that the integer build reads, after the
+ * blocks holding only exact zeros (values below their row's resolution, in
+ * every B_Q) are left out: out[0] the 16-column segments and out[1] the
+ * 8-column slots of the lower triangle that the Coulomb pair walks, out[2]
+ * the 64-column slabs of 32 rows per function that the half-transform
+ * multiplies (1.0 where a form is absent). No counterpart in NOPT, whose
+ * GEMMs read every element; a diagnostic of the kept forms. */
+int nopt_jk_zero_blocks(const nopt_jk_ctx* ctx, double out[3]);
+
 /* Active-space integrals through the fitted B, FP64 (the MO_INTS half of
  * calc_2el_MO_INTS_RI, src/RI.cpp:197-226):
  *   ints[(a2*na + a1)*na*na + b2*na + b1] = sum_Q R_Q[a2][a1] R_Q[b2][b1],

@@ -54,6 +54,7 @@ GPU_SYMBOLS = [
     ("nopt_jk_set_B_fitted", ctypes.c_int, [_vp, _c_double_p]),
     ("nopt_jk_get_B", ctypes.c_int, [_vp, _c_double_p]),
     ("nopt_jk_device_bytes", ctypes.c_long, [_vp]),
+    ("nopt_jk_zero_blocks", ctypes.c_int, [_vp, _c_double_p]),
     ("nopt_jk_active_ints", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, _c_double_p]),
     ("nopt_jk_build", ctypes.c_int, [_vp, _c_double_p, _c_double_p, ctypes.c_long, _c_double_p,
                                      _c_double_p, ctypes.c_int]),

@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnoptjk.so")
 
-HOST_SRCS = ["host/basis.cpp", "host/ints.cpp", "host/errors.cpp", "host/mol_capi.cpp"]
+HOST_SRCS = ["host/basis.cpp", "host/ints.cpp", "host/errors.cpp", "host/mol_capi.cpp", "host/order.cpp"]
 # host code that calls the GPU entry points (left out of a --host-only build)
 GLUE_SRCS = ["host/ri_shim.cpp", "host/ri_cxx.cpp"]
 HIP_SRCS = ["kernels/linalg.hip", "kernels/bf16.hip", "kernels/ozaki.hip", "kernels/jk.hip"]

@@ -31,6 +31,7 @@
 
 #include "../../../include/nopt_jk.h"
 #include "../host/errors.hpp"
+#include "../host/order.hpp"
 #include "common.hpp"
 #include "bf16.hpp"
 #include "linalg.hpp"
@@ -122,21 +123,23 @@ __global__ void k_active_ints(const double* __restrict__ R, long naux, int nn, d
   ints[idx] = s;
 }
 
-// dst[r][c] (ldd) = src[r][c] (lds) for r < rows, c < cols; zero elsewhere
-// in [0, rows_pad) x [0, ldd)
+// dst[r][c] (ldd) = src[r'][c'] (lds) for r < rows, c < cols; zero elsewhere
+// in [0, rows_pad) x [0, ldd); r' = rperm[r], c' = cperm[c] (each optional)
 template <typename T>
 __device__ __forceinline__ void k_pad_copy_body(const VB vb, const double* __restrict__ src, long lds, int rows,
-                                                int cols, T* __restrict__ dst, long ldd, int rows_pad) {
+                                                int cols, T* __restrict__ dst, long ldd, int rows_pad,
+                                                const int* __restrict__ rperm, const int* __restrict__ cperm) {
   const long e = (long)vb.x * blockDim.x + threadIdx.x;
   if (e >= (long)rows_pad * ldd) return;
   const long r = e / ldd, c = e % ldd;
-  double v = (r < rows && c < cols) ? src[r * lds + c] : 0.0;
+  double v = (r < rows && c < cols) ? src[(rperm ? rperm[r] : r) * lds + (cperm ? cperm[c] : c)] : 0.0;
   dst[e] = (T)v;
 }
 template <typename T>
 __global__ void k_pad_copy(const double* __restrict__ src, long lds, int rows, int cols,
-                           T* __restrict__ dst, long ldd, int rows_pad) {
-  k_pad_copy_body<T>(grid_vb(), src, lds, rows, cols, dst, ldd, rows_pad);
+                           T* __restrict__ dst, long ldd, int rows_pad, const int* __restrict__ rperm,
+                           const int* __restrict__ cperm) {
+  k_pad_copy_body<T>(grid_vb(), src, lds, rows, cols, dst, ldd, rows_pad, rperm, cperm);
 }
 // work-list form (FP64 copies)
 struct ItemPad {
@@ -147,13 +150,15 @@ struct ItemPad {
   double* dst;
   long ldd;
   int rows_pad;
+  const int* rperm;
+  const int* cperm;
   int gx, gy, gz;
 };
 __global__ void k_pad_copy_list(const ItemPad* __restrict__ items, const int* __restrict__ first, int n) {
   const int i = list_item(first, n, (int)blockIdx.x);
   const ItemPad& a = items[i];
   k_pad_copy_body<double>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz), a.src, a.lds, a.rows, a.cols, a.dst,
-                          a.ldd, a.rows_pad);
+                          a.ldd, a.rows_pad, a.rperm, a.cperm);
 }
 
 // --------------------------------------------------- half-transform f64/f32
@@ -534,32 +539,42 @@ __global__ __launch_bounds__(256) void k_syrk(const T* __restrict__ X, long ldx,
 
 // --------------------------------------------------------------- finalize
 // J = sum_s Jpart[s], K = sum_s Kpart[s] over the lower triangle, written to
-// both triangles of J, K (ld nbf). Either output may be null.
+// both triangles of J, K (ld nbf). Either output may be null. With jlive (the
+// integer Coulomb pair), J is zero in the 8-column slots (m, n/8) whose jlive
+// byte is 0: B is zero there for every Q, and the J kernels skip them.
 struct ItemFin {
   const double* Jpart;
   int nsj;
+  const uint8_t* jlive;
+  int nsr;
   const double* Kpart;
   int nsk;
   int ntiles;
   int kt;
   int nbf;
+  const int* perm;
   double* J;
   double* K;
   int gx, gy, gz;
 };
 __device__ __forceinline__ void k_finalize_body(const VB vb, const double* __restrict__ Jpart, int nsj,
+                                                const uint8_t* __restrict__ jlive, int nsr,
                                                 const double* __restrict__ Kpart, int nsk, int ntiles, int kt,
-                                                int nbf, double* __restrict__ J, double* __restrict__ K) {
+                                                int nbf, const int* __restrict__ perm, double* __restrict__ J,
+                                                double* __restrict__ K) {
   const int KT = kt;  // exchange tile edge of the partials
   const long e = (long)vb.x * blockDim.x + threadIdx.x;
   if (e >= (long)nbf * nbf) return;
   const int m = (int)(e / nbf), n = (int)(e % nbf);
   if (n > m) return;
+  // partials in the order of the integer forms (perm[new] = old)
+  const long pm = perm ? perm[m] : m, pn = perm ? perm[n] : n;
   if (J) {
     double s = 0.0;
-    for (int k = 0; k < nsj; ++k) s += Jpart[((long)k * nbf + m) * nbf + n];
-    J[(long)m * nbf + n] = s;
-    J[(long)n * nbf + m] = s;
+    if (!jlive || jlive[(long)m * nsr + n / 8])
+      for (int k = 0; k < nsj; ++k) s += Jpart[((long)k * nbf + m) * nbf + n];
+    J[pm * nbf + pn] = s;
+    J[pn * nbf + pm] = s;
   }
   if (K) {
     const int bi = m / KT, bj = n / KT;
@@ -567,21 +582,21 @@ __device__ __forceinline__ void k_finalize_body(const VB vb, const double* __res
     const int r = m % KT, c = n % KT;
     double s = 0.0;
     for (int k = 0; k < nsk; ++k) s += Kpart[(((long)k * ntiles + tile) * KT + r) * KT + c];
-    K[(long)m * nbf + n] = s;
-    K[(long)n * nbf + m] = s;
+    K[pm * nbf + pn] = s;
+    K[pn * nbf + pm] = s;
   }
 }
-__global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const double* __restrict__ Kpart,
-                           int nsk, int ntiles, int kt, int nbf, double* __restrict__ J,
-                           double* __restrict__ K) {
-  k_finalize_body(grid_vb(), Jpart, nsj, Kpart, nsk, ntiles, kt, nbf, J, K);
+__global__ void k_finalize(const double* __restrict__ Jpart, int nsj, const uint8_t* __restrict__ jlive, int nsr,
+                           const double* __restrict__ Kpart, int nsk, int ntiles, int kt, int nbf,
+                           const int* __restrict__ perm, double* __restrict__ J, double* __restrict__ K) {
+  k_finalize_body(grid_vb(), Jpart, nsj, jlive, nsr, Kpart, nsk, ntiles, kt, nbf, perm, J, K);
 }
 __global__ void k_finalize_list(const ItemFin* __restrict__ items,
     const int* __restrict__ first, int n) {
   const int i = list_item(first, n, (int)blockIdx.x);
   const ItemFin& a = items[i];
   k_finalize_body(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
-      a.Jpart, a.nsj, a.Kpart, a.nsk, a.ntiles, a.kt, a.nbf, a.J, a.K);
+      a.Jpart, a.nsj, a.jlive, a.nsr, a.Kpart, a.nsk, a.ntiles, a.kt, a.nbf, a.perm, a.J, a.K);
 }
 
 
@@ -634,6 +649,7 @@ constexpr size_t kFullRowsMax = (size_t)24 << 30;
 
 struct nopt_jk_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   long nbf = 0, naux = 0, npair = 0;
   long ld64 = 0;
   int mtiles = 0, ktn = 0, ntiles = 0;
@@ -695,8 +711,22 @@ struct nopt_jk_ctx {
   long splane = 0;
   int nseg = 0;
   int* segs = nullptr;   // segment table (m << 16 | n0), used to pack Bs
-  int* slots = nullptr;  // 8-column slots (m << 16 | n0) walked by the kernels
+  int* slots = nullptr;  // every 8-column slot (m << 16 | n0)
   int nslots = 0, nsg = 0, nsj8 = 0;
+  // what the kernels walk: the segments and slots in which some B_Q holds a
+  // nonzero value (all of them until B is formed), jlive[m*nsr + n/8] = 1
+  // for a live slot; and the half-transform's slab masks (naux x nmb words)
+  int* lsegs = nullptr;
+  int* lslots = nullptr;
+  int nlseg = 0, nlslots = 0, nsr = 0;
+  uint8_t* jlive = nullptr;
+  uint32_t* Bm = nullptr;
+  int nmb = 0;
+  // the order of the functions in B's integer forms (band_order of B's
+  // weights, so that exact-zero blocks gather): perm[new] = old, pinv its
+  // inverse; the integer builds read D and C and write J and K through it
+  int* perm = nullptr;
+  int* pinv = nullptr;
   // when the integer Coulomb pair starts on the side stream: 0 after the
   // half-transform; 1 gamma before it (beside it), J after it; 2 both before
   int cplace = 0;
@@ -857,6 +887,65 @@ int order_out(nopt_jk_ctx* c, hipStream_t s) {
   return NOPT_OK;
 }
 
+// Q-splits of the integer Coulomb pair: about four blocks per CU over the
+// slots the kernels walk.
+void coulomb_split_defaults(nopt_jk_ctx* c) {
+  const int nsb = nopt::gamma_slot_blocks(c->nlslots);
+  const int nbj = (c->nlslots + 256 * nopt::kJSPL - 1) / (256 * nopt::kJSPL);
+  c->nsg = (int)std::max(1L, std::min(c->naux, (4L * c->ncu + nsb - 1) / nsb));
+  c->nsj8 = (int)std::max(1L, std::min(c->naux, (4L * c->ncu + nbj - 1) / nbj));
+}
+
+// Once B's integer forms are written: the Coulomb segments and slots that
+// hold a nonzero value for some Q (the tables the kernels walk, and the J
+// map of the finalize), the half-transform's slab masks, and the Coulomb
+// split counts for the live tables.
+int find_live(nopt_jk_ctx* c) {
+  const long na = c->naux;
+  const int nbf = (int)c->nbf;
+  int* live = nullptr;
+  int rc = dev_alloc((void**)&live, sizeof(int) * c->nseg, "segment liveness");
+  if (rc) return rc;
+  auto run = [&]() -> int {
+    hipStream_t s = c->stream;
+    HIPCHK(hipMemsetAsync(live, 0, sizeof(int) * c->nseg, s));
+    HIPCHK(nopt::seg_live(c->Bs, c->splane, c->nseg, na, live, s));
+    if (c->Bm && c->Bd) HIPCHK(nopt::slab_mask_rows(c->Bd, na * c->nbf * c->ld8, c->ld8, nbf, na, c->Bm, s));
+    if (c->Bm && !c->Bd)
+      HIPCHK(nopt::slab_mask_packed(reinterpret_cast<const int*>(c->Bs), 16L * c->nseg, nbf, na, c->Bm, s));
+    std::vector<int> h((size_t)c->nseg);
+    HIPCHK(hipMemcpyAsync(h.data(), live, sizeof(int) * c->nseg, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<int> segs, slots;
+    std::vector<uint8_t> jl((size_t)nbf * c->nsr, 0);
+    long k = 0;  // segments in the order of coulomb_seg_table
+    for (int m = 0; m < nbf; ++m)
+      for (int n0 = 0; n0 <= m; n0 += 16, ++k) {
+        if (h[k]) segs.push_back((m << 16) | n0);
+        for (int half = 0; half < 2; ++half) {
+          const int n8 = n0 + 8 * half;
+          if (n8 <= m && (h[k] >> half & 1)) {
+            slots.push_back((m << 16) | n8);
+            jl[(size_t)m * c->nsr + n8 / 8] = 1;
+          }
+        }
+      }
+    if (segs.empty()) segs.push_back(0);  // B all zeros: one segment of zeros keeps the grids valid
+    if (slots.empty()) slots.push_back(0);
+    HIPCHK(hipMemcpyAsync(c->lsegs, segs.data(), sizeof(int) * segs.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->lslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->jlive, jl.data(), jl.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    c->nlseg = (int)segs.size();
+    c->nlslots = (int)slots.size();
+    coulomb_split_defaults(c);
+    return NOPT_OK;
+  };
+  rc = run();
+  dev_free(live);
+  return rc;
+}
+
 // Forms B from packed lower-triangle rows in host memory, host[P][m(m+1)/2+n]:
 // raw (P|mn) solved against the Cholesky factor dL of the metric (B = L^-1
 // (P|mn)), or an already fitted tensor when dL is null. The rows m stream
@@ -880,8 +969,20 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
     if (!rc) rc = dev_alloc((void**)&c->Be8, (size_t)na * c->ld8, "B row exponents (bytes)");
     c->splane = na * 16L * c->nseg;
     if (!rc) rc = dev_alloc((void**)&c->Bs, (size_t)5 * c->splane, "packed lower triangle of B");
+    c->nmb = (int)((nbf + 31) / 32);
+    if (!rc && c->ld8 <= 2048) rc = dev_alloc((void**)&c->Bm, sizeof(uint32_t) * na * c->nmb, "slab masks of B");
+    if (!rc) rc = dev_alloc((void**)&c->perm, sizeof(int) * nbf, "order of the functions");
+    if (!rc) rc = dev_alloc((void**)&c->pinv, sizeof(int) * nbf, "inverse order of the functions");
   }
   if (rc) return rc;
+  if (keep8) {
+    std::vector<int> hp((size_t)nbf), hi((size_t)nbf);
+    nopt::band_order(host, na, nbf, hp.data());
+    for (long i = 0; i < nbf; ++i) hi[(size_t)hp[(size_t)i]] = (int)i;
+    if (hipMemcpy(c->perm, hp.data(), sizeof(int) * nbf, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->pinv, hi.data(), sizeof(int) * nbf, hipMemcpyHostToDevice) != hipSuccess)
+      return set_error(NOPT_EHIP, "form_B: copy of the function order failed");
+  }
   double* dA = nullptr;
   unsigned long long* rmax = nullptr;
   rc = dev_alloc((void**)&dA, sizeof(double) * na * cp, "chunk of the packed tensor");
@@ -891,6 +992,7 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
     if (keep64) HIPCHK(hipMemsetAsync(c->B64, 0, sizeof(double) * rows * c->ld64, s));
     if (rmax) {
       if (c->Bd) HIPCHK(hipMemsetAsync(c->Bd, 0, (size_t)4 * rows * c->ld8, s));
+      HIPCHK(hipMemsetAsync(c->Bs, 0, (size_t)5 * c->splane, s));  // the scatter leaves the padding
       HIPCHK(hipMemsetAsync(rmax, 0, sizeof(unsigned long long) * rows, s));
     }
     // pass 1 forms the FP64 rows or the row maxima, pass 2 (integer forms
@@ -906,11 +1008,12 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
         if (keep64) {
           HIPCHK(nopt::unpack_packed_f64(dA, w, m0, m1, (int)nbf, na, c->B64, c->ld64, s));
         } else if (pass == 0) {
-          HIPCHK(nopt::rowmax_packed(dA, w, m0, m1, (int)nbf, na, rmax, s));
+          HIPCHK(nopt::rowmax_packed(dA, w, m0, m1, (int)nbf, na, rmax, s, c->pinv));
         } else {
-          HIPCHK(nopt::pack_segs_chunk(dA, w, m0, m1, (int)nbf, na, c->Be, c->segs, c->nseg, c->Bs, c->splane, s));
+          HIPCHK(nopt::scatter_segs_chunk(dA, w, m0, m1, (int)nbf, na, c->Be, c->pinv, c->nseg, c->Bs, c->splane, s));
           if (c->Bd)
-            HIPCHK(nopt::split_packed_i8(dA, w, m0, m1, (int)nbf, na, c->Be, 4, c->Bd, c->ld8, rows * c->ld8, s));
+            HIPCHK(nopt::split_packed_i8(dA, w, m0, m1, (int)nbf, na, c->Be, 4, c->Bd, c->ld8, rows * c->ld8, s,
+                                         c->pinv));
         }
         m0 = m1;
       }
@@ -919,8 +1022,9 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
     if (keep64 && keep8) {
       // the exponents (and digit planes) from the FP64 rows, then the packed triangle
       HIPCHK(nopt::split_rows_i8(c->B64, c->ld64, (int)nbf, rows, c->Bd ? 4 : 0, c->Bd, c->ld8, rows,
-                                 rows * c->ld8, c->Be, s));
-      HIPCHK(nopt::pack_segs_rows(c->B64, c->ld64, (int)nbf, na, c->Be, c->segs, c->nseg, c->Bs, c->splane, s));
+                                 rows * c->ld8, c->Be, s, c->perm, nbf, c->perm));
+      HIPCHK(nopt::pack_segs_rows(c->B64, c->ld64, (int)nbf, na, c->Be, c->segs, c->nseg, c->Bs, c->splane, s,
+                                  c->perm));
     }
     if (keep8) HIPCHK(nopt::expo_bytes(c->Be, (int)nbf, c->ld8, na, c->Be8, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -929,6 +1033,7 @@ int form_B(nopt_jk_ctx* c, const double* host, const double* dL) {
   if (!rc) rc = run();
   dev_free(dA);
   dev_free(rmax);
+  if (!rc && keep8) rc = find_live(c);
   if (!rc) c->b_ready = true;
   return rc;
 }
@@ -944,9 +1049,11 @@ struct BuildItem {
 };
 
 template <typename T>
-void stage_rows(const double* src, long lds, int rows, int cols, T* dst, long ldd, int rows_pad, hipStream_t s) {
+void stage_rows(const double* src, long lds, int rows, int cols, T* dst, long ldd, int rows_pad, hipStream_t s,
+                const int* rperm = nullptr, const int* cperm = nullptr) {
   const long n = (long)rows_pad * ldd;
-  if (n > 0) k_pad_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, lds, rows, cols, dst, ldd, rows_pad);
+  if (n > 0)
+    k_pad_copy<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(src, lds, rows, cols, dst, ldd, rows_pad, rperm, cperm);
 }
 
 // Work buffers of one build: the context's own (grown on demand) or carved
@@ -1191,7 +1298,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   wb.Dp = sizeof(double) * std::max(nj, 1) * c->dstride;
   wb.gamma = sizeof(double) * std::max(nj, 1) * naux;
   wb.gpart = sizeof(double) * naux * c->mtiles64;
-  wb.gslot = sizeof(double) * nopt::gamma_partial_blocks(c->nslots, c->nseg) * std::max(nj, 1) * naux;
+  wb.gslot = sizeof(double) * nopt::gamma_partial_blocks(c->nlslots, c->nlseg) * std::max(nj, 1) * naux;
   Work w;
   rc = work_area ? carve_work(work_area, work_area_bytes, wb, &w) : context_work(c, wb, &w);
   if (rc) return rc;
@@ -1206,30 +1313,39 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   // before the build ends, so that they overlap the exchange.
   const bool overlap = !rec && !tm && nj > 0 && anyK;
   const unsigned nelj = (unsigned)((nn + 255) / 256);  // finalize blocks
-  auto pad_rows = [&](int phase, const double* src, long lds, int r, int cols, double* dst, long ldd, int rpad) {
+  // the integer mode works on the functions in the order of B's integer
+  // forms (perm[new] = old): densities are read through it, orbitals'
+  // columns too, and J, K written back through it
+  const int* perm = i8 ? c->perm : nullptr;
+  auto pad_rows = [&](int phase, const double* src, long lds, int r, int cols, double* dst, long ldd, int rpad,
+                      const int* rperm = nullptr, const int* cperm = nullptr) {
     const long n = (long)rpad * ldd;
     if (n <= 0) return;
     if (rec)
       rec->add(phase, kListPad, (int)((n + 255) / 256), 256,
-               ItemPad{src, lds, r, cols, dst, ldd, rpad, (int)((n + 255) / 256), 1, 1});
+               ItemPad{src, lds, r, cols, dst, ldd, rpad, rperm, cperm, (int)((n + 255) / 256), 1, 1});
     else
-      stage_rows<double>(src, lds, r, cols, dst, ldd, rpad, s);
+      stage_rows<double>(src, lds, r, cols, dst, ldd, rpad, s, rperm, cperm);
   };
   auto split_rows = [&](const double* src, long ld, int cols, long rows_src, int8_t* planes, long ldo, long rows_dst,
-                        long pstride, int* expo) -> hipError_t {
-    if (!rec) return nopt::split_rows_i8(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, s);
+                        long pstride, int* expo, const int* cperm = nullptr) -> hipError_t {
+    if (!rec)
+      return nopt::split_rows_i8(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, s, nullptr, 0,
+                                 cperm);
     if (rows_dst <= 0) return hipSuccess;
     nopt::ListRec r;
-    CHK_(nopt::split_rows_item(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, &r));
+    CHK_(nopt::split_rows_item(src, ld, cols, rows_src, 4, planes, ldo, rows_dst, pstride, expo, &r, cperm));
     rec->add(1, r);
     return hipSuccess;
   };
   auto finalize = [&](int phase, const double* Jp, int nsj_, const double* Kp, int nsk_, int ntl, int kt,
                       double* J, double* K, hipStream_t st) {
+    const uint8_t* jl = Jp && i8 ? c->jlive : nullptr;  // J partials of the integer pair
     if (rec)
-      rec->add(phase, kListFin, (int)nelj, 256, ItemFin{Jp, nsj_, Kp, nsk_, ntl, kt, nbf, J, K, (int)nelj, 1, 1});
+      rec->add(phase, kListFin, (int)nelj, 256,
+               ItemFin{Jp, nsj_, jl, c->nsr, Kp, nsk_, ntl, kt, nbf, perm, J, K, (int)nelj, 1, 1});
     else
-      k_finalize<<<nelj, 256, 0, st>>>(Jp, nsj_, Kp, nsk_, ntl, kt, nbf, J, K);
+      k_finalize<<<nelj, 256, 0, st>>>(Jp, nsj_, jl, c->nsr, Kp, nsk_, ntl, kt, nbf, perm, J, K);
   };
   hipStream_t sj = overlap ? c->side : s;
   auto cmark = [&](int ph) {
@@ -1240,14 +1356,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       k_gamma<<<(naux + 255) / 256, 256, 0, sj>>>(w.gpart, c->mtiles, naux, w.gamma);
     } else if (i8 && rec) {
       nopt::ListRec r[2];
-      CHK_(nopt::gamma_slots_items(nj, c->nsg, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                   c->nslots, c->segs, c->nseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux, r));
+      CHK_(nopt::gamma_slots_items(nj, c->nsg, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->lslots,
+                                   c->nlslots, c->lsegs, c->nlseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux, r));
       rec->add(3, r[0]);
       rec->add(4, r[1]);
       return hipSuccess;
     } else if (i8) {
-      CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                    c->nslots, c->segs, c->nseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux));
+      CHK_(nopt::launch_gamma_slots(nj, c->nsg, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->lslots,
+                                    c->nlslots, c->lsegs, c->nlseg, w.Dp, ldb, c->dstride, w.gslot, w.gamma, naux));
     } else {
 #define NOPT_GF(ND)                                                                                         \
   case ND:                                                                                                  \
@@ -1263,14 +1379,14 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   auto coulomb_j = [&]() -> hipError_t {
     if (i8 && rec) {
       nopt::ListRec r;
-      CHK_(nopt::jpart_slots_item(nj, nsj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots, c->nslots,
-                                  c->segs, c->nseg, w.gamma, naux, Jpart, (long)nsj * nn, &r));
+      CHK_(nopt::jpart_slots_item(nj, nsj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->lslots,
+                                  c->nlslots, c->lsegs, c->nlseg, w.gamma, naux, Jpart, (long)nsj * nn, &r));
       rec->add(5, r);
       return hipSuccess;
     }
     if (i8)
-      return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->slots,
-                                      c->nslots, c->segs, c->nseg, w.gamma, naux, Jpart, (long)nsj * nn);
+      return nopt::launch_jpart_slots(nj, nsj, sj, c->Bs, c->splane, 16L * c->nseg, c->Be, nbf, naux, c->lslots,
+                                      c->nlslots, c->lsegs, c->nlseg, w.gamma, naux, Jpart, (long)nsj * nn);
     dim3 gj((nbf + 1) / 2, nsj);
 #define NOPT_JP(ND)                                                                                     \
   case ND:                                                                                              \
@@ -1294,7 +1410,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
   mark(0);
   // densities of the J list into slots 0..nj-1 (ld = ldb); orbital blocks
   // into the stacked factor at column offsets off[d] (zero rows between)
-  for (int k = 0; k < nj; ++k) pad_rows(0, it[jl[k]].D, nbf, nbf, nbf, w.Dp + k * c->dstride, ldb, nbf);
+  for (int k = 0; k < nj; ++k) pad_rows(0, it[jl[k]].D, nbf, nbf, nbf, w.Dp + k * c->dstride, ldb, nbf, perm, perm);
   if (anyK) {
     for (int d = 0; d < nd; ++d) {
       const int pad = (int)(off[d + 1] - off[d]);
@@ -1302,7 +1418,7 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       const int r = (int)it[d].nocc;
       if (i8)
         HIPCHK(split_rows(it[d].C, nbf, nbf, r, (int8_t*)w.Cd + off[d] * c->ld8, c->ld8, pad, ldx * c->ld8,
-                          (int*)w.Ce + off[d]));
+                          (int*)w.Ce + off[d], perm));
       else if (f64)
         stage_rows<double>(it[d].C, nbf, r, nbf, (double*)w.Cp + off[d] * ldb, ldb, pad, s);
       else
@@ -1357,7 +1473,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
           xs.nseg = k + 1;
         }
       if (xfused) gh.y = (unsigned)npq;
-      const nopt::HalfB hb{c->Bd, rows * c->ld8, reinterpret_cast<const int*>(c->Bs), 16L * c->nseg, c->Be8, c->Be};
+      const nopt::HalfB hb{c->Bd, rows * c->ld8, reinterpret_cast<const int*>(c->Bs), 16L * c->nseg, c->Be8, c->Be,
+                           c->Bm, c->nmb};
       if (rec) {
         nopt::ListRec r;
         e = nopt::half_i8_item(nt, qpb, gh, hb, c->ld8, nbf, naux, (const int8_t*)w.Cd, ldx * c->ld8,
@@ -1442,7 +1559,8 @@ int enqueue_batch(nopt_jk_ctx* c, int nd, const BuildItem* it, int prec, hipStre
       else
         HIPCHK(nopt::launch_syrk_bf16(prec == NOPT_PREC_BF16X3 ? 2 : 3, gk, s, (const float*)w.X + off[d], ldx,
                                       nbf, naux, kpad, w.Kpart));
-      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, w.Kpart, c->nsk, c->ntiles, KT, nbf, nullptr, it[d].K);
+      k_finalize<<<nel, 256, 0, s>>>(nullptr, 0, nullptr, 0, w.Kpart, c->nsk, c->ntiles, KT, nbf, nullptr, nullptr,
+                                     it[d].K);
     }
     HIPCHK(hipGetLastError());
   }
@@ -1603,21 +1721,28 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
     c->nslots = nopt::coulomb_slot_count((int)nbf);
     std::vector<int> tab((size_t)c->nslots);
     nopt::coulomb_slot_table((int)nbf, tab.data());
-    const int nsb = nopt::gamma_slot_blocks(c->nslots);
-    const int nbj = (c->nslots + 256 * nopt::kJSPL - 1) / (256 * nopt::kJSPL);
     int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu <= 0)
-      ncu = 256;
-    c->nsg = (int)std::max(1L, std::min((long)naux, (4L * ncu + nsb - 1) / nsb));
-    c->nsj8 = (int)std::max(1L, std::min((long)naux, (4L * ncu + nbj - 1) / nbj));
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+      c->ncu = ncu;
+    c->nlslots = c->nslots;
+    coulomb_split_defaults(c);
+    c->nsr = (int)((nbf + 7) / 8);
+    const std::vector<uint8_t> ones((size_t)nbf * c->nsr, 1);
     rc = dev_alloc((void**)&c->slots, sizeof(int) * c->nslots, "Coulomb slot table");
     if (!rc) rc = dev_alloc((void**)&c->segs, sizeof(int) * c->nseg, "Coulomb segment table");
+    if (!rc) rc = dev_alloc((void**)&c->lslots, sizeof(int) * c->nslots, "Coulomb live slot table");
+    if (!rc) rc = dev_alloc((void**)&c->lsegs, sizeof(int) * c->nseg, "Coulomb live segment table");
+    if (!rc) rc = dev_alloc((void**)&c->jlive, ones.size(), "Coulomb live slot map");
     if (!rc)
       rc = dev_alloc((void**)&c->gslot,
                      sizeof(double) * nopt::gamma_partial_blocks(c->nslots, c->nseg) * kMaxBatch * naux,
                      "gamma partials");
+    c->nlseg = c->nseg;
     if (!rc && (hipMemcpy(c->slots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(c->segs, stab.data(), sizeof(int) * c->nseg, hipMemcpyHostToDevice) != hipSuccess))
+                hipMemcpy(c->segs, stab.data(), sizeof(int) * c->nseg, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->lslots, tab.data(), sizeof(int) * c->nslots, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->lsegs, stab.data(), sizeof(int) * c->nseg, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->jlive, ones.data(), ones.size(), hipMemcpyHostToDevice) != hipSuccess))
       rc = set_error(NOPT_EHIP, "nopt_jk_create: copy of the slot tables failed");
   }
   if (rc) {
@@ -1664,6 +1789,12 @@ void nopt_jk_destroy(nopt_jk_ctx* c) {
   dev_free(c->Xe);
   dev_free(c->slots);
   dev_free(c->segs);
+  dev_free(c->lslots);
+  dev_free(c->lsegs);
+  dev_free(c->jlive);
+  dev_free(c->Bm);
+  dev_free(c->perm);
+  dev_free(c->pinv);
   dev_free(c->gslot);
   dev_free(c->Bs);
   for (int i = 0; i < kMaxBatch - 1; ++i) {
@@ -2051,12 +2182,31 @@ int nopt_jk_active_ints(nopt_jk_ctx* c, const double* C1, const double* C2, long
   return NOPT_OK;
 }
 
+int nopt_jk_zero_blocks(const nopt_jk_ctx* c, double out[3]) {
+  if (!c || !out) return set_error(NOPT_EINVAL, "nopt_jk_zero_blocks: null argument");
+  out[0] = out[1] = out[2] = 1.0;
+  if (!c->b_ready || !c->Bs) return NOPT_OK;
+  out[0] = (double)c->nlseg / c->nseg;
+  out[1] = (double)c->nlslots / c->nslots;
+  if (c->Bm) {
+    std::vector<uint32_t> h((size_t)c->naux * c->nmb);
+    if (hipSetDevice(c->device) != hipSuccess ||
+        hipMemcpy(h.data(), c->Bm, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      return set_error(NOPT_EHIP, "nopt_jk_zero_blocks: copy of the slab masks failed");
+    long bits = 0;
+    for (uint32_t v : h) bits += __builtin_popcount(v);
+    out[2] = (double)bits / ((double)h.size() * (double)(c->ld8 / 64));
+  }
+  return NOPT_OK;
+}
+
 long nopt_jk_device_bytes(const nopt_jk_ctx* c) {
   if (!c) return 0;
   long b = 0;
   if (c->B64) b += sizeof(double) * c->naux * c->nbf * c->ld64;
   if (c->Bd) b += 4L * c->naux * c->nbf * c->ld8;
   if (c->Bs) b += sizeof(int) * c->naux * c->nbf + c->naux * c->ld8 + 5L * c->splane;
+  if (c->Bm) b += sizeof(uint32_t) * c->naux * c->nmb;
   b += (long)(c->X_bytes + c->Cp_bytes + c->Cd_bytes + c->Ce_bytes + c->Xd_bytes + c->Xe_bytes + c->Jpart_bytes);
   b += (long)c->Kpart_bytes;
   b += sizeof(double) * (c->naux * c->mtiles64 + kMaxBatch * c->naux + kMaxBatch * c->dstride + 5 * c->nbf * c->nbf);

@@ -72,16 +72,27 @@ __device__ __forceinline__ void digits5(double x, int e, int& n28, int& d4) {
   d4 = (int)rint(r);
 }
 
+// first 16-column segment of row m in the packed lower triangle Bs (row r
+// holds r/16 + 1 segments; the layout is described with k_pack_segs)
+__host__ __device__ __forceinline__ long seg0(int m) {
+  const long a = m / 16, b = m % 16;
+  return m + 8 * a * (a - 1) + a * b;
+}
+
 // Splits the rows of a row-major FP64 matrix (rows_src x cols, ld) into ns
 // int8 digit planes planes[k*pstride + row*ldo + col] (rows_dst rows, zero
-// padded) and row exponents. One wavefront per row.
+// padded) and row exponents. One wavefront per row. Optional reorders:
+// output row r reads source row (r/rper)*rper + rperm[r%rper], output column
+// c source column cperm[c].
 __device__ __forceinline__ void k_split_rows_body(const VB vb, const double* __restrict__ src, long ld, int cols,
                                                    long rows_src, int ns, int8_t* __restrict__ planes,
-                                                   long ldo, long pstride, int* __restrict__ expo) {
+                                                   long ldo, long pstride, int* __restrict__ expo,
+                                                   const int* __restrict__ rperm, long rper,
+                                                   const int* __restrict__ cperm) {
   const long row = vb.x;
   const int lane = threadIdx.x;
   const bool live = row < rows_src;
-  const double* s = src + row * ld;
+  const double* s = src + (rperm ? row / rper * rper + rperm[row % rper] : row) * ld;
   double mx = 0.0;
   if (live)
     for (int c = lane; c < cols; c += 64) mx = fmax(mx, fabs(s[c]));
@@ -89,7 +100,7 @@ __device__ __forceinline__ void k_split_rows_body(const VB vb, const double* __r
   const int e = row_exponent(mx);
   if (lane == 0) expo[row] = e;
   for (int c = lane; c < ldo; c += 64) {
-    double r = (live && c < cols) ? ldexp(s[c], 6 - e) : 0.0;  // |r| < 64
+    double r = (live && c < cols) ? ldexp(s[cperm ? cperm[c] : c], 6 - e) : 0.0;  // |r| < 64
     for (int k = 0; k < ns; ++k) {
       const double d = rint(r);
       planes[k * pstride + row * ldo + c] = (int8_t)d;
@@ -99,15 +110,17 @@ __device__ __forceinline__ void k_split_rows_body(const VB vb, const double* __r
 }
 __global__ __launch_bounds__(64) void k_split_rows(const double* __restrict__ src, long ld, int cols,
                                                    long rows_src, int ns, int8_t* __restrict__ planes,
-                                                   long ldo, long pstride, int* __restrict__ expo) {
-  k_split_rows_body(grid_vb(), src, ld, cols, rows_src, ns, planes, ldo, pstride, expo);
+                                                   long ldo, long pstride, int* __restrict__ expo,
+                                                   const int* __restrict__ rperm, long rper,
+                                                   const int* __restrict__ cperm) {
+  k_split_rows_body(grid_vb(), src, ld, cols, rows_src, ns, planes, ldo, pstride, expo, rperm, rper, cperm);
 }
 __global__ __launch_bounds__(64) void k_split_rows_list(const ItemSplit* __restrict__ items,
     const int* __restrict__ first, int n) {
   const int i = list_item(first, n, (int)blockIdx.x);
   const ItemSplit& a = items[i];
   k_split_rows_body(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
-      a.src, a.ld, a.cols, a.rows_src, a.ns, a.planes, a.ldo, a.pstride, a.expo);
+      a.src, a.ld, a.cols, a.rows_src, a.ns, a.planes, a.ldo, a.pstride, a.expo, nullptr, 0, a.cperm);
 }
 
 
@@ -119,13 +132,21 @@ __global__ __launch_bounds__(64) void k_split_rows_list(const ItemSplit* __restr
 // (m, n) and its mirror (n, m) are both written along contiguous columns.
 // Modes: 0 writes FP64 rows (out64, ld), 1 folds |x| into the row maxima
 // (bit patterns of non-negative doubles order like their values), 2 writes
-// ns int8 digits with the rows' exponents (the digits k_split_rows forms).
+// ns int8 digits with the rows' exponents (the digits k_split_rows forms),
+// 3 writes the packed lower triangle Bs (planes = Bs, pstride = splane,
+// ld = sq) of the reordered functions: the element (m, n) goes to row
+// max(a, b), column min(a, b) with a, b = pinv[m], pinv[n], as digits5 of the
+// value at that row's exponent. With pinv, modes 1 and 2 write the row and
+// column pinv[m], pinv[n] as well (the function m of B is function pinv[m]
+// of the kept forms).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_packed_tiles(const double* __restrict__ A, long cp, int m0, int m1, int nbf,
                                                       double* __restrict__ out64, long ld,
                                                       unsigned long long* __restrict__ rowmax,
                                                       const int* __restrict__ expo, int ns,
-                                                      int8_t* __restrict__ planes, long pstride) {
+                                                      int8_t* __restrict__ planes, long pstride,
+                                                      const int* __restrict__ pinv) {
+  auto fn = [&](int i) { return pinv ? pinv[i] : i; };
   __shared__ double tile[64][65];
   const int nb = blockIdx.x * 64, mb = m0 + blockIdx.y * 64;
   if (nb > mb + 63) return;  // above the diagonal: nothing stored (uniform per block)
@@ -145,24 +166,34 @@ __global__ __launch_bounds__(256) void k_packed_tiles(const double* __restrict__
   for (int k = 0; k < 16; ++k) {
     const int m = mb + w + 4 * k, n = nb + c;
     const bool ok = m < m1 && n <= m;
-    const long row = P * nbf + m;
     if constexpr (MODE == 0) {
-      if (ok) out64[row * ld + n] = v[k];
+      if (ok) out64[(P * nbf + m) * ld + n] = v[k];
     } else if constexpr (MODE == 1) {
       double mx = fabs(v[k]);
       for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-      if (c == 0 && m < m1) atomicMax(rowmax + row, (unsigned long long)__double_as_longlong(mx));
-    } else {
+      if (c == 0 && m < m1) atomicMax(rowmax + P * nbf + fn(m), (unsigned long long)__double_as_longlong(mx));
+    } else if constexpr (MODE == 2) {
       if (ok) {
+        const long row = P * nbf + fn(m);
         double r = ldexp(v[k], 6 - expo[row]);
         for (int d = 0; d < ns; ++d) {
           const double dg = rint(r);
-          planes[d * pstride + row * ld + n] = (int8_t)dg;
+          planes[d * pstride + row * ld + fn(n)] = (int8_t)dg;
           r = (r - dg) * 128.0;
         }
       }
+    } else {
+      if (ok) {
+        const int a = fn(m), b = fn(n), r = max(a, b), cc = min(a, b);
+        int n28, d4;
+        digits5(v[k], expo[P * nbf + r], n28, d4);
+        const long el = P * ld + 16 * seg0(r) + cc;
+        reinterpret_cast<int*>(planes)[el] = n28;
+        planes[4 * pstride + el] = (int8_t)d4;
+      }
     }
   }
+  if constexpr (MODE == 3) return;  // each stored element is one element of the triangle
   __syncthreads();
   // the mirror: row n = nb + rr, column m = mb + c (strictly below the diagonal)
 #pragma unroll
@@ -170,19 +201,19 @@ __global__ __launch_bounds__(256) void k_packed_tiles(const double* __restrict__
     const int rr = w + 4 * k, n = nb + rr, m = mb + c;
     const bool ok = m < m1 && n < m;
     const double x = ok ? tile[c][rr] : 0.0;
-    const long row = P * nbf + n;
     if constexpr (MODE == 0) {
-      if (ok) out64[row * ld + m] = x;
+      if (ok) out64[(P * nbf + n) * ld + m] = x;
     } else if constexpr (MODE == 1) {
       double mx = fabs(x);
       for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-      if (c == 0 && n < nbf) atomicMax(rowmax + row, (unsigned long long)__double_as_longlong(mx));
-    } else {
+      if (c == 0 && n < nbf) atomicMax(rowmax + P * nbf + fn(n), (unsigned long long)__double_as_longlong(mx));
+    } else if constexpr (MODE == 2) {
       if (ok) {
+        const long row = P * nbf + fn(n);
         double r = ldexp(x, 6 - expo[row]);
         for (int d = 0; d < ns; ++d) {
           const double dg = rint(r);
-          planes[d * pstride + row * ld + m] = (int8_t)dg;
+          planes[d * pstride + row * ld + fn(m)] = (int8_t)dg;
           r = (r - dg) * 128.0;
         }
       }
@@ -208,32 +239,31 @@ __global__ void k_rowmax_exponent(const unsigned long long* __restrict__ rowmax,
 // bytes), and the signed byte d4 at Bs[4*splane + e]. The Coulomb kernels
 // rebuild N35 exactly as fma(N28, 128, d4); the half-transform reads N28
 // alone.
-__host__ __device__ __forceinline__ long seg0(int m) {
-  const long a = m / 16, b = m % 16;
-  return m + 8 * a * (a - 1) + a * b;
-}
 
 // Bs for the segments [s0, s1) (the rows of one chunk) with the row exponents
-// Be: SRC 0 from FP64 rows B[(Q*nbf + m)*ld + n], SRC 1 from a packed chunk
-// A[Q*cp + m(m+1)/2 - m0(m0+1)/2 + n] of the rows m0 <= m < m1. Grid:
-// (s1 - s0)/256 x naux, one segment per thread.
+// Be: SRC 0 from FP64 rows B[(Q*nbf + m)*ld + n] (with perm, the row and
+// column m, n of Bs read the row and column perm[m], perm[n] of B), SRC 1
+// from a packed chunk A[Q*cp + m(m+1)/2 - m0(m0+1)/2 + n] of the rows
+// m0 <= m < m1. Grid: (s1 - s0)/256 x naux, one segment per thread.
 template <int SRC>
 __global__ __launch_bounds__(256) void k_pack_segs(const double* __restrict__ A, long ld, int m0, int nbf,
                                                    const int* __restrict__ Be, const int* __restrict__ seg_code,
-                                                   int s0, int s1, long sq, int8_t* __restrict__ Bs, long splane) {
+                                                   int s0, int s1, long sq, int8_t* __restrict__ Bs, long splane,
+                                                   const int* __restrict__ perm) {
   const int s = s0 + blockIdx.x * 256 + threadIdx.x;
   if (s >= s1) return;
   const long Q = blockIdx.y;
   const int code = seg_code[s], m = code >> 16, n0 = code & 0xffff;
   const int e = Be[Q * nbf + m];
-  const double* src = SRC == 0 ? A + (Q * nbf + m) * ld + n0
-                               : A + Q * ld + (long)m * (m + 1) / 2 - (long)m0 * (m0 + 1) / 2 + n0;
+  const double* src = SRC == 0 ? A + (Q * nbf + (perm ? perm[m] : m)) * ld
+                               : A + Q * ld + (long)m * (m + 1) / 2 - (long)m0 * (m0 + 1) / 2;
   int part[16];
   uint32_t low[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     int n28 = 0, d4 = 0;
-    if (n0 + j <= m) digits5(src[j], e, n28, d4);
+    const int n = n0 + j;
+    if (n <= m) digits5(src[SRC == 0 && perm ? perm[n] : n], e, n28, d4);
     part[j] = n28;
     low[j / 4] |= (uint32_t)(d4 & 255) << (8 * (j % 4));
   }
@@ -251,6 +281,84 @@ __global__ void k_expo8(const int* __restrict__ Be, int nbf, long ld8, long naux
   if (i >= naux * ld8) return;
   const long Q = i / ld8, n = i % ld8;
   Be8[i] = n < nbf ? (int8_t)Be[Q * nbf + n] : (int8_t)0;
+}
+
+// ------------------------------------------------------ blocks of exact zeros
+// Where B's integer forms hold nothing but zeros. An element whose value lies
+// below the resolution of its row (2^-28 of the row maximum in the digit
+// planes, 2^-35 in Bs) is stored as an exact zero, and for a molecule of some
+// extent most pairs (m, n) of distant functions are such zeros in every B_Q.
+// The kernels leave these blocks out: adding zero products changes no bit of
+// an exact integer sum, and a J element that is zero in every B_Q is zero.
+
+// live[s] |= bit 0 (bit 1) when the first (last) eight columns of Coulomb
+// segment s hold a nonzero N28 or d4 for some function in the range of
+// blockIdx.y. Grid: nseg/256 x function ranges; live zeroed before.
+__global__ __launch_bounds__(256) void k_seg_live(const int8_t* __restrict__ Bs, long splane, long sq, int nseg,
+                                                  int naux, int* __restrict__ live) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= nseg) return;
+  const int q0 = (int)((long)naux * blockIdx.y / gridDim.y), q1 = (int)((long)naux * (blockIdx.y + 1) / gridDim.y);
+  uint32_t bits = 0;
+  for (int q = q0; q < q1 && bits != 3u; ++q) {
+    const long e = (long)q * sq + 16L * s;
+    const uint4* p = reinterpret_cast<const uint4*>(reinterpret_cast<const int*>(Bs) + e);
+    const uint4 a = p[0], b = p[1], c = p[2], d = p[3];
+    const uint4 l = *reinterpret_cast<const uint4*>(Bs + 4 * splane + e);
+    if ((a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w | l.x | l.y) != 0u) bits |= 1u;
+    if ((c.x | c.y | c.z | c.w | d.x | d.y | d.z | d.w | l.z | l.w) != 0u) bits |= 2u;
+  }
+  if (bits) atomicOr(live + s, (int)bits);
+}
+
+// Bm[Q*nmb + mb] bit k: the rows 32 mb .. 32 mb + 31 of function Q hold a
+// nonzero digit in the columns 64 k .. 64 k + 63 of the full-row planes Bd
+// (4 planes of stride bplane, rows of ld8 bytes). Grid: nmb x naux.
+__global__ __launch_bounds__(256) void k_slab_mask_rows(const int8_t* __restrict__ Bd, long bplane, long ld8, int nbf,
+                                                        int nmb, uint32_t* __restrict__ Bm) {
+  __shared__ uint32_t red;
+  const int mb = blockIdx.x;
+  const long Q = blockIdx.y;
+  if (threadIdx.x == 0) red = 0u;
+  __syncthreads();
+  const int rows = min(32, nbf - 32 * mb), cpr = (int)(ld8 / 16), per = rows * cpr;
+  uint32_t bits = 0;
+  for (int i = threadIdx.x; i < 4 * per; i += 256) {
+    const int p = i / per, r = (i % per) / cpr, c = i % cpr;
+    const uint4 v = *reinterpret_cast<const uint4*>(Bd + p * bplane + (Q * nbf + 32 * mb + r) * ld8 + 16 * c);
+    if ((v.x | v.y | v.z | v.w) != 0u) bits |= 1u << (c / 4);
+  }
+  if (bits) atomicOr(&red, bits);
+  __syncthreads();
+  if (threadIdx.x == 0) Bm[Q * nmb + mb] = red;
+}
+
+// The same masks for the packed loader, from the N28 of Bs (int32 plane Bp,
+// sq values per function): (m, n <= m) from row m, (m, n > m) from the stored
+// (n, m), which the loader rescales (a zero stays zero, a nonzero value counts
+// as live). The second loop walks the stored rows n so that consecutive
+// threads read consecutive columns m. Grid: nmb x naux.
+__global__ __launch_bounds__(256) void k_slab_mask_packed(const int* __restrict__ Bp, long sq, int nbf, int nmb,
+                                                          uint32_t* __restrict__ Bm) {
+  __shared__ uint32_t red;
+  const int mb = blockIdx.x, mlo = 32 * mb;
+  const long Q = blockIdx.y;
+  if (threadIdx.x == 0) red = 0u;
+  __syncthreads();
+  const int rows = min(32, nbf - mlo);
+  const int* B = Bp + Q * sq;
+  uint32_t bits = 0;
+  for (int i = threadIdx.x; i < rows * nbf; i += 256) {
+    const int m = mlo + i / nbf, n = i % nbf;
+    if (n <= m && B[16 * seg0(m) + n] != 0) bits |= 1u << (n / 64);
+  }
+  for (int i = threadIdx.x; i < nbf * 32; i += 256) {
+    const int n = i / 32, m = mlo + i % 32;
+    if (m < mlo + rows && n > m && B[16 * seg0(n) + m] != 0) bits |= 1u << (n / 64);
+  }
+  if (bits) atomicOr(&red, bits);
+  __syncthreads();
+  if (threadIdx.x == 0) Bm[Q * nmb + mb] = red;
 }
 
 // ------------------------------------------------------------ half-transform
@@ -287,7 +395,7 @@ __device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >>
 template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
 __device__ __forceinline__ void k_half_i8_body(const VB vb, 
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Bp, long sq, const int8_t* __restrict__ Be8,
-    const int* __restrict__ Be, long ld8, int nbf, int naux,
+    const int* __restrict__ Be, const uint32_t* __restrict__ Bm, int nmb, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit& xs) {
   constexpr int BN = 16 * NT, KS = 64, NWT = NW * QPB, NTH = 64 * NWT;
@@ -502,18 +610,42 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
     for (int j = 0; j < JT; ++j)
 #pragma unroll
       for (int c = 0; c < 5; ++c) acc[i][j][c] = v4i_t{0, 0, 0, 0};
+  // the k-slabs to multiply: every slab (ld8 <= 2048: at most 32) less those
+  // in which the block's rows of B hold no nonzero digit (Bm: a 32-bit mask
+  // per function and 32 rows); a zero slab adds nothing to the class sums,
+  // so leaving it out changes no bit of X
   const int nks = (int)(ld8 / KS);
-  NOPT_GLOAD_A(0);
-  if constexpr (PK) lstore_packed(sm[0]);
-  NOPT_GLOAD_C(0);
-  NOPT_LSTORE(0);
-  __syncthreads();
+  uint32_t todo = nks >= 32 ? 0xffffffffu : (1u << nks) - 1u;
+  if (Bm) {
+    uint32_t any = 0;
+#pragma unroll
+    for (int q = 0; q < QPB; ++q)
+#pragma unroll
+      for (int h = 0; h < BMI / 32; ++h) {
+        const int mb = m0 / 32 + h;
+        if (Qb + q < naux && mb < nmb) any |= Bm[(Qb + q) * nmb + mb];
+      }
+    todo &= any;
+  }
+  todo = __builtin_amdgcn_readfirstlane(todo);
+  bool have = todo != 0;
+  if (have) {
+    const int k0 = __builtin_ctz(todo) * KS;
+    todo &= todo - 1u;
+    NOPT_GLOAD_A(k0);
+    if constexpr (PK) lstore_packed(sm[0]);
+    NOPT_GLOAD_C(k0);
+    NOPT_LSTORE(0);
+    __syncthreads();
+  }
   const int lr = lane & 15, lc = lane >> 4;
-  for (int ks = 0; ks < nks; ++ks) {
+  for (int ks = 0; have; ++ks) {
     const int cur = ks & 1;
+    const bool more = todo != 0;
+    const int kn = more ? __builtin_ctz(todo) * KS : 0;  // the next slab
     // the B slab (HBM) is fetched before the products, the small C slab
     // (L2-resident) after them (measured faster than fetching both first)
-    if (ks + 1 < nks) NOPT_GLOAD_A((ks + 1) * KS);
+    if (more) NOPT_GLOAD_A(kn);
     const int8_t* As = sm[cur] + qq * AB;
     const int8_t* Cs = sm[cur] + QPB * AB;
     uint4 av[IT][4];
@@ -562,14 +694,16 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
         A[4] = mfma_i8(av[i][1], bv, A[4]);
       }
     }
-    if (ks + 1 < nks) {
+    if (more) {
       // packed: the B slab is converted and stored before the C slab is
       // fetched (fewer registers live at once)
       if constexpr (PK) lstore_packed(sm[cur ^ 1]);
-      NOPT_GLOAD_C((ks + 1) * KS);
+      NOPT_GLOAD_C(kn);
       NOPT_LSTORE(cur ^ 1);
     }
     __syncthreads();
+    todo &= todo - 1u;
+    have = more;
   }
 #undef NOPT_GLOAD_A
 #undef NOPT_GLOAD_C
@@ -737,11 +871,11 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
 template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
 __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_half_i8(
     const int8_t* __restrict__ Bd, long bplane, const int* __restrict__ Bp, long sq, const int8_t* __restrict__ Be8,
-    const int* __restrict__ Be, long ld8, int nbf, int naux,
+    const int* __restrict__ Be, const uint32_t* __restrict__ Bm, int nmb, long ld8, int nbf, int naux,
     const int8_t* __restrict__ Cd, long cplane, const int* __restrict__ Ce, double* __restrict__ X, long ldx,
     const XSplit xs) {
   k_half_i8_body<NT, WC, NW, QPB, BMI, WPE, PK>(grid_vb(),
-      Bd, bplane, Bp, sq, Be8, Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs);
+      Bd, bplane, Bp, sq, Be8, Be, Bm, nmb, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs);
 }
 template <int NT, int WC, int NW, int QPB, int BMI, int WPE, bool PK>
 __global__ __launch_bounds__(64 * NW * QPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE)))
@@ -750,7 +884,8 @@ void k_half_i8_list(const ItemHalf* __restrict__ items,
   const int i = list_item(first, n, (int)blockIdx.x);
   const ItemHalf& a = items[i];
   k_half_i8_body<NT, WC, NW, QPB, BMI, WPE, PK>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
-      a.Bd, a.bplane, a.Bp, a.sq, a.Be8, a.Be, a.ld8, a.nbf, a.naux, a.Cd, a.cplane, a.Ce, a.X, a.ldx, a.xs);
+      a.Bd, a.bplane, a.Bp, a.sq, a.Be8, a.Be, a.Bm, a.nmb, a.ld8, a.nbf, a.naux, a.Cd, a.cplane, a.Ce, a.X, a.ldx,
+      a.xs);
 }
 
 
@@ -1268,10 +1403,13 @@ void k_syrk_i8_list(const ItemSyrk* __restrict__ items,
 }  // namespace
 
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                         long rows_dst, long pstride, int* expo, hipStream_t s) {
+                         long rows_dst, long pstride, int* expo, hipStream_t s, const int* rperm, long rper,
+                         const int* cperm) {
   if (rows_dst <= 0) return hipSuccess;
-  if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo) return hipErrorInvalidValue;
-  k_split_rows<<<(unsigned)rows_dst, 64, 0, s>>>(src, ld, cols, rows_src, ns, planes, ldo, pstride, expo);
+  if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo || (rperm && (rper <= 0 || rows_src % rper)))
+    return hipErrorInvalidValue;
+  k_split_rows<<<(unsigned)rows_dst, 64, 0, s>>>(src, ld, cols, rows_src, ns, planes, ldo, pstride, expo, rperm,
+                                                 rper, cperm);
   return hipGetLastError();
 }
 
@@ -1287,15 +1425,15 @@ hipError_t unpack_packed_f64(const double* A, long cp, int m0, int m1, int nbf, 
                              hipStream_t s) {
   if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || ld < nbf) return hipErrorInvalidValue;
   k_packed_tiles<0><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, B, ld, nullptr, nullptr, 0,
-                                                               nullptr, 0);
+                                                               nullptr, 0, nullptr);
   return hipGetLastError();
 }
 
 hipError_t rowmax_packed(const double* A, long cp, int m0, int m1, int nbf, long naux, unsigned long long* rowmax,
-                         hipStream_t s) {
+                         hipStream_t s, const int* pinv) {
   if (!packed_chunk_ok(cp, m0, m1, nbf, naux)) return hipErrorInvalidValue;
   k_packed_tiles<1><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, nullptr, 0, rowmax, nullptr, 0,
-                                                               nullptr, 0);
+                                                               nullptr, 0, pinv);
   return hipGetLastError();
 }
 
@@ -1305,11 +1443,20 @@ hipError_t rowmax_exponents(const unsigned long long* rowmax, long rows, int* ex
 }
 
 hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* expo, int ns,
-                           int8_t* planes, long ldo, long pstride, hipStream_t s) {
+                           int8_t* planes, long ldo, long pstride, hipStream_t s, const int* pinv) {
   if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || ldo < nbf || ns < 1 || ns > 8) return hipErrorInvalidValue;
   if (pstride < naux * nbf * ldo) return hipErrorInvalidValue;
   k_packed_tiles<2><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, nullptr, ldo, nullptr, expo, ns,
-                                                               planes, pstride);
+                                                               planes, pstride, pinv);
+  return hipGetLastError();
+}
+
+hipError_t scatter_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* Be,
+                              const int* pinv, int nseg, int8_t* Bs, long splane, hipStream_t s) {
+  if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || nseg != seg0(nbf) || splane < naux * 16L * nseg)
+    return hipErrorInvalidValue;
+  k_packed_tiles<3><<<packed_grid(m0, m1, naux), 256, 0, s>>>(A, cp, m0, m1, nbf, nullptr, 16L * nseg, nullptr, Be,
+                                                               0, Bs, splane, pinv);
   return hipGetLastError();
 }
 
@@ -1381,8 +1528,9 @@ hipError_t launch_half_i8(int nt, int qpb, dim3 grid, hipStream_t s, const HalfB
   case half_cfg(PK_, NT_, QPB_):                                                                          \
     k_half_i8<NT_, WC_, NW_, QPB_, BMI_, WPE_, PK_><<<dim3((nbf + BMI_ - 1) / BMI_, grid.y, grid.z),       \
                                                      64 * NW_ * QPB_, 0, s>>>(b.Bd, b.bplane, b.Bp, b.sq,  \
-                                                                              b.Be8, b.Be, ld8, nbf, naux, \
-                                                                              Cd, cplane, Ce, X, ldx, xs); \
+                                                                              b.Be8, b.Be, b.Bm, b.nmb, ld8, \
+                                                                              nbf, naux, Cd, cplane, Ce, X, \
+                                                                              ldx, xs);                     \
     break;
   switch (half_cfg(!b.Bd, nt, qpb)) {
     NOPT_HALF_CONFIGS(NOPT_HALF_I8)
@@ -1401,10 +1549,10 @@ void coulomb_seg_table(int nbf, int* out) {
 }
 
 hipError_t pack_segs_rows(const double* B, long ld, int nbf, long naux, const int* Be, const int* seg_code, int nseg,
-                          int8_t* Bs, long splane, hipStream_t s) {
+                          int8_t* Bs, long splane, hipStream_t s, const int* perm) {
   if (naux <= 0 || naux > 65535 || splane < naux * 16L * nseg || ld < nbf) return hipErrorInvalidValue;
   k_pack_segs<0><<<dim3((nseg + 255) / 256, (unsigned)naux), 256, 0, s>>>(B, ld, 0, nbf, Be, seg_code, 0, nseg,
-                                                                         16L * nseg, Bs, splane);
+                                                                         16L * nseg, Bs, splane, perm);
   return hipGetLastError();
 }
 
@@ -1413,13 +1561,36 @@ hipError_t pack_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, lo
   if (!packed_chunk_ok(cp, m0, m1, nbf, naux) || splane < naux * 16L * nseg) return hipErrorInvalidValue;
   const int s0 = (int)seg0(m0), s1 = (int)seg0(m1);
   k_pack_segs<1><<<dim3((s1 - s0 + 255) / 256, (unsigned)naux), 256, 0, s>>>(A, cp, m0, nbf, Be, seg_code, s0, s1,
-                                                                             16L * nseg, Bs, splane);
+                                                                             16L * nseg, Bs, splane, nullptr);
   return hipGetLastError();
 }
 
 hipError_t expo_bytes(const int* Be, int nbf, long ld8, long naux, int8_t* Be8, hipStream_t s) {
   if (ld8 < nbf) return hipErrorInvalidValue;
   k_expo8<<<(unsigned)((naux * ld8 + 255) / 256), 256, 0, s>>>(Be, nbf, ld8, naux, Be8);
+  return hipGetLastError();
+}
+
+hipError_t seg_live(const int8_t* Bs, long splane, int nseg, long naux, int* live, hipStream_t s) {
+  if (nseg <= 0 || naux <= 0 || splane < naux * 16L * nseg) return hipErrorInvalidValue;
+  const unsigned ranges = (unsigned)std::min(naux, 64L);
+  k_seg_live<<<dim3((nseg + 255) / 256, ranges), 256, 0, s>>>(Bs, splane, 16L * nseg, nseg, (int)naux, live);
+  return hipGetLastError();
+}
+
+hipError_t slab_mask_rows(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, uint32_t* Bm,
+                          hipStream_t s) {
+  if (ld8 % 64 || ld8 > 2048 || ld8 < nbf || naux <= 0 || naux > 65535 || bplane < naux * nbf * ld8)
+    return hipErrorInvalidValue;
+  const int nmb = (nbf + 31) / 32;
+  k_slab_mask_rows<<<dim3(nmb, (unsigned)naux), 256, 0, s>>>(Bd, bplane, ld8, nbf, nmb, Bm);
+  return hipGetLastError();
+}
+
+hipError_t slab_mask_packed(const int* Bp, long sq, int nbf, long naux, uint32_t* Bm, hipStream_t s) {
+  if (nbf > 2048 || naux <= 0 || naux > 65535 || sq < 16 * seg0(nbf)) return hipErrorInvalidValue;
+  const int nmb = (nbf + 31) / 32;
+  k_slab_mask_packed<<<dim3(nmb, (unsigned)naux), 256, 0, s>>>(Bp, sq, nbf, nmb, Bm);
   return hipGetLastError();
 }
 
@@ -1472,13 +1643,13 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
 
 // ------------------------------------------------------------- work lists
 hipError_t split_rows_item(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                           long rows_dst, long pstride, int* expo, ListRec* r) {
+                           long rows_dst, long pstride, int* expo, ListRec* r, const int* cperm) {
   if (rows_src > rows_dst || cols > ldo || pstride < rows_dst * ldo || rows_dst <= 0) return hipErrorInvalidValue;
   r->kind = kListSplit;
   r->cfg = 0;
   r->blocks = (int)rows_dst;
   r->threads = 64;
-  put_args(r, ItemSplit{src, ld, cols, rows_src, ns, planes, ldo, pstride, expo, (int)rows_dst, 1, 1});
+  put_args(r, ItemSplit{src, ld, cols, rows_src, ns, planes, ldo, pstride, expo, cperm, (int)rows_dst, 1, 1});
   return hipSuccess;
 }
 
@@ -1493,8 +1664,8 @@ hipError_t half_i8_item(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, in
   r->cfg = half_cfg(pk, nt, qpb);
   r->blocks = gx * (int)grid.y * (int)grid.z;
   r->threads = half_i8_threads(nt, qpb, pk);
-  put_args(r, ItemHalf{b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, ld8, nbf, naux, Cd, cplane, Ce, X, ldx, xs, gx,
-                       (int)grid.y, (int)grid.z});
+  put_args(r, ItemHalf{b.Bd, b.bplane, b.Bp, b.sq, b.Be8, b.Be, b.Bm, b.nmb, ld8, nbf, naux, Cd, cplane, Ce, X, ldx,
+                       xs, gx, (int)grid.y, (int)grid.z});
   return hipSuccess;
 }
 

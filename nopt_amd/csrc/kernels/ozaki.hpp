@@ -8,8 +8,11 @@ namespace nopt {
 // Rows of src (rows_src x cols, ld) -> ns digit planes
 // planes[k*pstride + row*ldo + col] (rows_dst >= rows_src rows, extra rows and
 // columns zero) and one exponent per row.
+// Optional reorders: output row r reads source row (r/rper)*rper +
+// rperm[r%rper], output column c source column cperm[c].
 hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                         long rows_dst, long pstride, int* expo, hipStream_t s);
+                         long rows_dst, long pstride, int* expo, hipStream_t s, const int* rperm = nullptr,
+                         long rper = 0, const int* cperm = nullptr);
 // Full rows from a chunk of a symmetric tensor kept as packed lower
 // triangles, A[P*cp + m(m+1)/2 + n - m0(m0+1)/2] for the rows m0 <= m < m1
 // (n <= m) of every P < naux: each element goes to both full rows it belongs
@@ -17,14 +20,16 @@ hipError_t split_rows_i8(const double* src, long ld, int cols, long rows_src, in
 // rowmax_packed folds the chunk into the row maxima (bit patterns, zeroed by
 // the caller), rowmax_exponents turns them into exponents, and
 // split_packed_i8 writes the digit planes of split_rows_i8 with those
-// exponents. Padding columns are the caller's to clear.
+// exponents. Padding columns are the caller's to clear. With pinv, the
+// function m of the chunk is function pinv[m] of the rows written (rows and
+// columns both reordered).
 hipError_t unpack_packed_f64(const double* A, long cp, int m0, int m1, int nbf, long naux, double* B, long ld,
                              hipStream_t s);
 hipError_t rowmax_packed(const double* A, long cp, int m0, int m1, int nbf, long naux, unsigned long long* rowmax,
-                         hipStream_t s);
+                         hipStream_t s, const int* pinv = nullptr);
 hipError_t rowmax_exponents(const unsigned long long* rowmax, long rows, int* expo, hipStream_t s);
 hipError_t split_packed_i8(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* expo, int ns,
-                           int8_t* planes, long ldo, long pstride, hipStream_t s);
+                           int8_t* planes, long ldo, long pstride, hipStream_t s, const int* pinv = nullptr);
 // Digit split of the half-transformed rows fused into the half-transform:
 // segment d covers the columns [off[d], off[d] + len[d]) of X (off a multiple
 // of 16); its rows get their own exponent xe[d][row] and four digit planes
@@ -51,6 +56,11 @@ struct HalfB {
   long sq;
   const int8_t* Be8;
   const int* Be;
+  // optional: Bm[Q*nmb + mb] bit k set when the rows 32 mb .. 32 mb + 31 of
+  // function Q hold a nonzero digit in the columns 64 k .. 64 k + 63, as the
+  // kernel reads them (slab_mask_rows / slab_mask_packed); null: every slab
+  const uint32_t* Bm = nullptr;
+  int nmb = 0;
 };
 
 // Half-transform X = B C^T (FP64 out) from digit planes of B and C
@@ -82,12 +92,28 @@ constexpr int kJSPL = 2;      // slots per thread of the J kernel
 int coulomb_seg_count(int nbf);
 void coulomb_seg_table(int nbf, int* out);
 hipError_t pack_segs_rows(const double* B, long ld, int nbf, long naux, const int* Be, const int* seg_code, int nseg,
-                          int8_t* Bs, long splane, hipStream_t s);
+                          int8_t* Bs, long splane, hipStream_t s, const int* perm = nullptr);
 hipError_t pack_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* Be,
                            const int* seg_code, int nseg, int8_t* Bs, long splane, hipStream_t s);
+// pack_segs_rows with perm: Bs of the reordered functions (row and column m
+// of Bs read row and column perm[m] of B); scatter_segs_chunk: the same from
+// a packed chunk, each stored element to its place in the reordered
+// triangle (pinv = the inverse order; Bs zeroed before the first chunk).
+hipError_t scatter_segs_chunk(const double* A, long cp, int m0, int m1, int nbf, long naux, const int* Be,
+                              const int* pinv, int nseg, int8_t* Bs, long splane, hipStream_t s);
 hipError_t expo_bytes(const int* Be, int nbf, long ld8, long naux, int8_t* Be8, hipStream_t s);
 int coulomb_slot_count(int nbf);
 void coulomb_slot_table(int nbf, int* out);
+// Blocks of exact zeros (values below their row's resolution are stored as
+// zeros). seg_live: live[s] |= 1 (2) when the first (last) eight columns of
+// segment s hold a nonzero N28 or d4 for some Q (live zeroed before); the
+// Coulomb kernels then walk tables of the live segments and slots only.
+// slab_mask_rows / slab_mask_packed: the half-transform's slab masks (HalfB::Bm)
+// from the full-row planes or from the N28 of Bs, nmb = ceil(nbf/32) words
+// per function.
+hipError_t seg_live(const int8_t* Bs, long splane, int nseg, long naux, int* live, hipStream_t s);
+hipError_t slab_mask_rows(const int8_t* Bd, long bplane, long ld8, int nbf, long naux, uint32_t* Bm, hipStream_t s);
+hipError_t slab_mask_packed(const int* Bp, long sq, int nbf, long naux, uint32_t* Bm, hipStream_t s);
 int gamma_slot_blocks(int nslots);
 // One or two densities walk the 16-column segments instead (one 16-byte load
 // per plane and lane); gamma_partial_blocks bounds the partials of either.
@@ -128,6 +154,7 @@ struct ItemSplit {
   long ldo;
   long pstride;
   int* expo;
+  const int* cperm;
   int gx, gy, gz;
 };
 struct ItemHalf {
@@ -137,6 +164,8 @@ struct ItemHalf {
   long sq;
   const int8_t* Be8;
   const int* Be;
+  const uint32_t* Bm;
+  int nmb;
   long ld8;
   int nbf;
   int naux;
@@ -200,7 +229,7 @@ struct ItemSyrk {
   int gx, gy, gz;
 };
 hipError_t split_rows_item(const double* src, long ld, int cols, long rows_src, int ns, int8_t* planes, long ldo,
-                           long rows_dst, long pstride, int* expo, ListRec* r);
+                           long rows_dst, long pstride, int* expo, ListRec* r, const int* cperm = nullptr);
 hipError_t half_i8_item(int nt, int qpb, dim3 grid, const HalfB& b, long ld8, int nbf, int naux, const int8_t* Cd,
                         long cplane, const int* Ce, double* X, long ldx, const XSplit& xs, ListRec* r);
 // two recs: the gamma partials and their sum

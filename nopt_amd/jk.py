@@ -103,6 +103,14 @@ class JKContext:
     def device_bytes(self):
         return int(L.lib().nopt_jk_device_bytes(self._h))
 
+    def zero_blocks(self):
+        """Shares of B's integer forms the integer build reads once the blocks
+        of exact zeros are left out: Coulomb segments, Coulomb slots, and
+        half-transform slabs (nopt_jk_zero_blocks)."""
+        out = np.zeros(3)
+        L.check(L.lib().nopt_jk_zero_blocks(self._h, L.dptr(out)))
+        return dict(segments=float(out[0]), slots=float(out[1]), slabs=float(out[2]))
+
     def build(self, D=None, C=None, nocc=None, prec="fp64", J=True, K=True):
         """(J, K) for density D (nbf x nbf) and the first nocc rows of C."""
         n = self.nbf
