@@ -77,7 +77,7 @@ def time_builds(ctx, dD, dC, nocc, dJ, dK, prec, steps, warmup, synchronize, tim
 # kernels of each timed phase, per mode
 KERNELS = {
     "half_transform": {"fp64": "k_half<double, 6>", "fp32": "k_half<float, 6>", "bf16x6": "k_half_bf16<3, 6>",
-                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 2, 4, 2, 32, 4>"},
+                       "bf16x3": "k_half_bf16<2, 6>", "int8": "k_half_i8<6, 2, 4, 2, 32, 4, false>"},
     "gamma": {"fp64": "k_gamma (fused partials)", "fp32": "k_gamma (fused partials)",
               "bf16x6": "k_gamma (fused partials)", "bf16x3": "k_gamma (fused partials)",
               "int8": "k_gamma_slots<1, 2, true>"},
@@ -88,14 +88,20 @@ KERNELS = {
 }
 
 
-def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
+def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False, shares=None):
     """Algorithmic work of the phase's kernel per launch, over its HIP-event time.
     Flops are the FP64 contraction's (F_half = 2 Q nbf^2 o, F_K = Q o nbf(nbf+1),
     one triangle), whatever arithmetic carries them, against the peak of the
     arithmetic used; the int8 modes' digit-pair products (13 per product) are
     reported beside as the emulation's own rate. Bytes count each operand
     element once at its stored width (no padding): B in the form the kernel
-    reads (full rows or packed lower triangle), C, and X written."""
+    reads (full rows or packed lower triangle), C, and X written. The int8
+    kernels leave out B's blocks of exact zeros: their B bytes and the
+    half-transform's products count the share they read (`shares`, from
+    nopt_jk_zero_blocks: slabs for the half-transform, segments for the
+    Coulomb pair), and `dense_equivalent` gives the rate counted over all of B."""
+    sh = shares or {}
+    s_slab, s_seg = sh.get("slabs", 1.0), sh.get("segments", 1.0)
     t = ms * 1e-3
     npair = nbf * (nbf + 1) / 2
 
@@ -114,13 +120,19 @@ def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
     if phase == "half_transform":
         if prec == "fp64":
             return mfma(f_half, PEAK["fp64_TF"])
-        if prec == "int8":  # 4 B digit planes in, 4 X digit planes out
-            r = hbm(4.0 * naux * (npair if half_packed else nbf * nbf) + 4.0 * naux * nbf * nocc)
+        if prec == "int8":  # 4 B digit planes in (the slabs read), 4 X digit planes out
+            bb = 4.0 * naux * (npair if half_packed else nbf * nbf)
+            r = hbm(s_slab * bb + 4.0 * naux * nbf * nocc)
             r["b_form"] = ("packed lower triangle (28-bit integers, unpacked to 4 digit planes in LDS)"
                            if half_packed else "4 int8 digit planes, full rows")
-            r["mfma_view"] = dict(algorithmic_flops=f_half, achieved_TOPs=f_half / t / 1e12,
-                                  frac_of_int8_peak=f_half / t / 1e12 / PEAK["int8_TOPS"],
-                                  emulation_frac=13 * f_half / t / 1e12 / PEAK["int8_TOPS"])
+            r["share_of_B_read"] = s_slab
+            r["dense_equivalent"] = dict(bytes=bb + 4.0 * naux * nbf * nocc,
+                                         achieved_GBps=(bb + 4.0 * naux * nbf * nocc) / t / 1e9)
+            fl = s_slab * f_half
+            r["mfma_view"] = dict(algorithmic_flops=fl, achieved_TOPs=fl / t / 1e12,
+                                  frac_of_int8_peak=fl / t / 1e12 / PEAK["int8_TOPS"],
+                                  emulation_frac=13 * fl / t / 1e12 / PEAK["int8_TOPS"],
+                                  dense_equivalent_TOPs=f_half / t / 1e12)
             return r
         return hbm(8.0 * naux * nbf * nbf + 4.0 * naux * nbf * nocc)  # FP64 B in, FP32 X out
     if phase == "exchange":
@@ -136,12 +148,16 @@ def roofline(phase, ms, nbf, naux, nocc, prec, half_packed=False):
         pk = {"fp64": PEAK["fp64_TF"], "fp32": PEAK["fp32_TF"], "bf16x6": PEAK["bf16_TF"] / 6,
               "bf16x3": PEAK["bf16_TF"] / 3}[prec]
         return mfma(f_k, pk)
+    if phase in ("gamma", "coulomb_j") and prec == "int8":
+        # lower triangle: int32 high plane + signed-byte low digit, the live segments
+        r = hbm(5.0 * naux * npair * s_seg)
+        r["share_of_B_read"] = s_seg
+        r["dense_equivalent"] = dict(bytes=5.0 * naux * npair, achieved_GBps=5.0 * naux * npair / t / 1e9)
+        return r
     if phase == "gamma":
-        if prec == "int8":  # lower triangle: int32 high plane + signed-byte low digit
-            return hbm(5.0 * naux * npair)
         return None  # partials ride on the half-transform
     if phase == "coulomb_j":
-        return hbm((5.0 if prec == "int8" else 8.0) * naux * npair)
+        return hbm(8.0 * naux * npair)
     return None
 
 
@@ -633,9 +649,10 @@ def main():
 
     timed = ("half_transform", "gamma", "coulomb_j", "exchange")
     hp = args.compact
-    dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec, hp)), key=lambda k: phases[k])
-    roof = roofline(dom, phases[dom], nbf, naux, nocc, prec, hp)
-    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p, hp) for k in timed} for p in modes}
+    zb = zero_blocks
+    dom = max((k for k in timed if roofline(k, phases[k], nbf, naux, nocc, prec, hp, zb)), key=lambda k: phases[k])
+    roof = roofline(dom, phases[dom], nbf, naux, nocc, prec, hp, zb)
+    kern = {p: {k: roofline(k, runs[p][2][k], nbf, naux, nocc, p, hp, zb) for k in timed} for p in modes}
     # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated) for this workload
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic_c20h42.json")
     if args.alkane == 20 and os.path.exists(tfile):

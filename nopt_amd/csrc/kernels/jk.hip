@@ -888,12 +888,12 @@ int order_out(nopt_jk_ctx* c, hipStream_t s) {
 }
 
 // Q-splits of the integer Coulomb pair: about four blocks per CU over the
-// slots the kernels walk.
+// slots the kernels walk (at most 32 J partials, which the output sums).
 void coulomb_split_defaults(nopt_jk_ctx* c) {
   const int nsb = nopt::gamma_slot_blocks(c->nlslots);
   const int nbj = (c->nlslots + 256 * nopt::kJSPL - 1) / (256 * nopt::kJSPL);
   c->nsg = (int)std::max(1L, std::min(c->naux, (4L * c->ncu + nsb - 1) / nsb));
-  c->nsj8 = (int)std::max(1L, std::min(c->naux, (4L * c->ncu + nbj - 1) / nbj));
+  c->nsj8 = (int)std::max(1L, std::min({c->naux, 32L, (4L * c->ncu + nbj - 1) / nbj}));
 }
 
 // Once B's integer forms are written: the Coulomb segments and slots that
