@@ -361,6 +361,19 @@ __global__ __launch_bounds__(256) void k_slab_mask_packed(const int* __restrict_
   if (threadIdx.x == 0) Bm[Q * nmb + mb] = red;
 }
 
+// Maximum over each row of 16 lanes, in every lane of the row: rotations by
+// 8, 4, 2 and 1 within the row (DPP row_ror, register moves).
+template <int N>
+__device__ __forceinline__ int ror16(int x) {
+  return __builtin_amdgcn_update_dpp(0, x, 0x120 + N, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_max16(int x) {
+  x = max(x, ror16<8>(x));
+  x = max(x, ror16<4>(x));
+  x = max(x, ror16<2>(x));
+  return max(x, ror16<1>(x));
+}
+
 // ------------------------------------------------------------ half-transform
 // LDS position of 16-byte chunk c (of a 64-byte k-slab) in row r: an XOR
 // swizzle under which the 16 rows x 4 chunks of one MFMA operand read, and
@@ -715,6 +728,18 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
   // 2^28 x the digit sum: hi = 128 A0 + A1 is exact in int32 (|A0| <= 4096 ld8,
   // ld8 <= 2048), so is mid = 128 A2 + A3 for ld8 <= 1024 (|A2| <= 3 * 4096 ld8),
   // and every partial sum below 2^52 is exact in FP64
+  // the row and column exponents, all loads issued at once (indices clamped
+  // into range; rows beyond nbf and a Q beyond naux give zeros below)
+  int ebe[IT][4], ece[JT];
+  {
+    const long Qc = Q < naux ? Q : naux - 1;
+#pragma unroll
+    for (int i = 0; i < IT; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ebe[i][v] = Be[Qc * nbf + min(m0 + rowl(i, v), nbf - 1)];
+#pragma unroll
+    for (int j = 0; j < JT; ++j) ece[j] = Ce[cg * BN + colb(j)];
+  }
   auto combine = [&](auto MID) {
 #pragma unroll
     for (int i = 0; i < IT; ++i)
@@ -722,7 +747,7 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
       for (int v = 0; v < 4; ++v) {
         const int m = m0 + rowl(i, v);
         const bool live = m < nbf && Q < naux;
-        const int em = live ? Be[Q * nbf + m] : 0;
+        const int em = ebe[i][v];
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
           const v4i_t* A = acc[i][j];
@@ -732,7 +757,7 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
             sv = fma((double)hi, 0x1p21, fma((double)(A[2][v] * 128 + A[3][v]), 128.0, (double)A[4][v]));
           else
             sv = fma((double)hi, 0x1p21, fma((double)A[2][v], 0x1p14, fma((double)A[3][v], 128.0, (double)A[4][v])));
-          xv[i][j][v] = live ? ldexp(sv, em + Ce[cg * BN + colb(j)] - 40) : 0.0;
+          xv[i][j][v] = live ? ldexp(sv, em + ece[j] - 40) : 0.0;
         }
       }
   };
@@ -796,8 +821,7 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
           (void)frexp(x, &k);
           if (c0 >= o && c0 < o + pad && x != 0.0) ex = max(ex, k);
         }
-#pragma unroll
-        for (int sh = 1; sh < 16; sh <<= 1) ex = max(ex, __shfl_xor(ex, sh, 64));
+        ex = row_max16(ex);
         if constexpr (kRegExp)
           rx[i][v] = ex;
         else if (lr == 0)
