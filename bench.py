@@ -84,7 +84,7 @@ KERNELS = {
     "coulomb_j": {"fp64": "k_jpart<1>", "fp32": "k_jpart<1>", "bf16x6": "k_jpart<1>", "bf16x3": "k_jpart<1>",
                   "int8": "k_jpart_slots<1, 2, true>"},
     "exchange": {"fp64": "k_syrk<double>", "fp32": "k_syrk<float>", "bf16x6": "k_syrk_bf16<3>",
-                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8<4>"},
+                 "bf16x3": "k_syrk_bf16<2>", "int8": "k_syrk_i8<4, true>"},
 }
 
 

@@ -668,45 +668,33 @@ __device__ __forceinline__ void k_half_i8_body(const VB vb,
 #pragma unroll
       for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * BMI + r) * KS + 16 * swz(r, lc));
     }
-    // one digit plane l of the C operand at a time: pairs (k, l), k+l <= 4
-#pragma unroll
-    for (int j = 0; j < JT; ++j) {
+    // one digit plane l of the C operand at a time: pairs (k, l), k+l <= 4;
+    // each plane's LDS read is issued before the products of the previous
+    // plane, so that its latency hides behind them (two operand registers
+    // in turn; the int32 sums are exact in any order)
+    auto cplane_ptr = [&](int j, int l) {
       const int col = wc * 16 * JT + j * 16 + lr;
-      const int8_t* cp = Cs + col * KS + 16 * swz(col, lc);
-      uint4 bv = *reinterpret_cast<const uint4*>(cp);
+      return reinterpret_cast<const uint4*>(Cs + col * KS + 16 * swz(col, lc) + l * BN * KS);
+    };
+    uint4 bq[2];
+    bq[0] = *cplane_ptr(0, 0);
+    // scheduling groups (DS read 0x100, MFMA 0x008): the operand reads first,
+    // then per plane one read of the next plane and the plane's products (not
+    // for the packed loader, whose conversion leaves no registers to spare)
+    if constexpr (!PK) __builtin_amdgcn_sched_group_barrier(0x100, 4 * IT + 1, 0);
+    static_for<4 * JT>([&](auto S) {
+      constexpr int s_ = decltype(S)::value, j = s_ / 4, l = s_ % 4;
+      if constexpr (s_ + 1 < 4 * JT) bq[(s_ + 1) & 1] = *cplane_ptr((s_ + 1) / 4, (s_ + 1) % 4);
+      const uint4 bv = bq[s_ & 1];
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
         v4i_t* A = acc[i][j];
-        A[0] = mfma_i8(av[i][0], bv, A[0]);
-        A[1] = mfma_i8(av[i][1], bv, A[1]);
-        A[2] = mfma_i8(av[i][2], bv, A[2]);
-        A[3] = mfma_i8(av[i][3], bv, A[3]);
-      }
-      bv = *reinterpret_cast<const uint4*>(cp + BN * KS);
 #pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[1] = mfma_i8(av[i][0], bv, A[1]);
-        A[2] = mfma_i8(av[i][1], bv, A[2]);
-        A[3] = mfma_i8(av[i][2], bv, A[3]);
-        A[4] = mfma_i8(av[i][3], bv, A[4]);
+        for (int k = 0; k + l <= 4 && k < 4; ++k) A[k + l] = mfma_i8(av[i][k], bv, A[k + l]);
       }
-      bv = *reinterpret_cast<const uint4*>(cp + 2 * BN * KS);
-#pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[2] = mfma_i8(av[i][0], bv, A[2]);
-        A[3] = mfma_i8(av[i][1], bv, A[3]);
-        A[4] = mfma_i8(av[i][2], bv, A[4]);
-      }
-      bv = *reinterpret_cast<const uint4*>(cp + 3 * BN * KS);
-#pragma unroll
-      for (int i = 0; i < IT; ++i) {
-        v4i_t* A = acc[i][j];
-        A[3] = mfma_i8(av[i][0], bv, A[3]);
-        A[4] = mfma_i8(av[i][1], bv, A[4]);
-      }
-    }
+      if constexpr (!PK && s_ + 1 < 4 * JT) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      if constexpr (!PK) __builtin_amdgcn_sched_group_barrier(0x008, IT * (l < 2 ? 4 : 5 - l), 0);
+    });
     if (more) {
       // packed: the B slab is converted and stored before the C slab is
       // fetched (fewer registers live at once)
@@ -1233,7 +1221,10 @@ constexpr int KT8 = 64;
 // robin) owns a contiguous range of Q-splits with all their tiles: the tiles
 // of one split read the same X rows at about the same time, from that XCD's
 // L2. The grid is 1-D, ntiles * nsplit blocks, nsplit a multiple of 8.
-template <int NW>
+// BUF: the operand and exponent loads as buffer loads (32-bit offsets: a
+// digit plane and the exponents below 2^31 bytes, which the launchers check),
+// else as loads through 64-bit addresses.
+template <int NW, bool BUF>
 __device__ __forceinline__ void k_syrk_i8_body(const VB vb, 
     const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
     int ntiles, int nsplit, double* __restrict__ Kpart) {
@@ -1255,49 +1246,79 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
   // loader: row (t/4)%64 of operand tile (t/256 + o) (o < OPT; tile 0 rows
   // bi, tile 1 rows bj), chunk t%4, all 4 planes
   const int ar = (t >> 2) & 63, ac = t & 3, op0 = (OPT == 2) ? 0 : (t >> 8);
-  const int8_t* pr[OPT];
+  // loads from a base address the same for the whole block (the slab's,
+  // in scalar registers) plus the lane's row and chunk as a 32-bit offset;
+  // rows beyond nbf load row nbf - 1 and store zeros, so every lane issues
+  // the same loads and the waits count them exactly
+  // rows beyond nbf load row nbf - 1 and store zeros: every lane issues the
+  // same loads, so that the waits count them exactly. Buffer form: a
+  // resource per digit plane, the lane's row and chunk as the vector offset,
+  // the slab's offset (the same for the whole block) as the scalar one.
+  __amdgpu_buffer_rsrc_t rX[4];
+  if constexpr (BUF)
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+      rX[p] = __builtin_amdgcn_make_buffer_rsrc(const_cast<int8_t*>(Xd + p * xplane), (short)0, (int)xplane,
+                                                0x00020000);
+  long voff[OPT];
   bool rok[OPT];
 #pragma unroll
   for (int o = 0; o < OPT; ++o) {
     const int m = ((op0 + o) == 0 ? bi : bj) * KT8 + ar;
     rok[o] = m < nbf;
-    pr[o] = Xd + (long)m * ldk + 16 * ac;
+    voff[o] = (long)min(m, nbf - 1) * ldk + 16 * ac;
   }
   const long qstride = (long)nbf * ldk;
   uint4 o_reg[OPT][4];
   auto gload = [&](int sl) {
     const int q = q0 + sl / nkq, k0 = (sl % nkq) * KS;
-    const long off = (long)q * qstride + k0;
+    if constexpr (BUF) {
+      const int so = __builtin_amdgcn_readfirstlane((int)(q * qstride + k0));
 #pragma unroll
-    for (int o = 0; o < OPT; ++o)
+      for (int o = 0; o < OPT; ++o)
 #pragma unroll
-      for (int p = 0; p < 4; ++p)
-        o_reg[o][p] = rok[o] ? *reinterpret_cast<const uint4*>(pr[o] + p * xplane + off) : uint4{0, 0, 0, 0};
+        for (int p = 0; p < 4; ++p) {
+          const auto x = __builtin_amdgcn_raw_buffer_load_b128(rX[p], (int)voff[o], so, 0);
+          o_reg[o][p] = uint4{x[0], x[1], x[2], x[3]};
+        }
+    } else {
+      const int8_t* base = Xd + q * qstride + k0;
+#pragma unroll
+      for (int o = 0; o < OPT; ++o)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) o_reg[o][p] = *reinterpret_cast<const uint4*>(base + p * xplane + voff[o]);
+    }
   };
   auto lstore = [&](int buf) {
 #pragma unroll
     for (int o = 0; o < OPT; ++o) {
       int8_t* Ts = sm[buf] + (op0 + o) * TB;
 #pragma unroll
-      for (int p = 0; p < 4; ++p) *reinterpret_cast<uint4*>(Ts + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) = o_reg[o][p];
+      for (int p = 0; p < 4; ++p)
+        *reinterpret_cast<uint4*>(Ts + (p * KT8 + ar) * KS + 16 * swz(ar, ac)) =
+            rok[o] ? o_reg[o][p] : uint4{0, 0, 0, 0};
     }
   };
   // exponents of this thread's accumulator rows (IT x 4) and columns (2)
   int ea[IT][4], eb[2];
+  __amdgpu_buffer_rsrc_t rXe;
+  if constexpr (BUF)
+    rXe = __builtin_amdgcn_make_buffer_rsrc(const_cast<int*>(Xe), (short)0, (int)(4L * naux * nbf), 0x00020000);
   auto eload = [&](int q, int (&xa)[IT][4], int (&xb)[2]) {
-    const int* e = Xe + (long)q * nbf;
+    // rows beyond nbf read row nbf - 1 (their products are zero)
+    auto ld = [&](int m) {
+      if constexpr (BUF)
+        return (int)__builtin_amdgcn_raw_buffer_load_b32(rXe, 4 * min(m, nbf - 1),
+                                                         __builtin_amdgcn_readfirstlane(4 * q * nbf), 0);
+      else
+        return Xe[(long)q * nbf + min(m, nbf - 1)];
+    };
 #pragma unroll
     for (int i = 0; i < IT; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int m = bi * KT8 + wr * 16 * IT + i * 16 + 4 * lc + v;
-        xa[i][v] = m < nbf ? e[m] : 0;
-      }
+      for (int v = 0; v < 4; ++v) xa[i][v] = ld(bi * KT8 + wr * 16 * IT + i * 16 + 4 * lc + v);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int n = bj * KT8 + wc * 32 + j * 16 + lr;
-      xb[j] = n < nbf ? e[n] : 0;
-    }
+    for (int j = 0; j < 2; ++j) xb[j] = ld(bj * KT8 + wc * 32 + j * 16 + lr);
   };
   // a wave whose block lies wholly above the diagonal of a diagonal tile
   // computes nothing there (k_finalize reads only the lower triangle)
@@ -1321,9 +1342,14 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
     const int cur = sl & 1;
     const bool more = sl + 1 < nsl;
     const bool qend = (sl % nkq) == nkq - 1;
-    if (more) gload(sl + 1);
-    // the exponents of this function's rows, needed after its last slab
+    // the exponents of this function's rows, needed after its last slab,
+    // requested before the next slab, so that waiting for them leaves the
+    // slab's loads in flight
     if (qend) eload(q0 + sl / nkq, ea, eb);
+    // (the last slab loads itself again, unused: every path then issues
+    // the same loads, and the wait for the exponents leaves the slab's in
+    // flight)
+    gload(more ? sl + 1 : sl);
     const int8_t* As = sm[cur];
     const int8_t* Bs = sm[cur] + TB;
     if (!upper) {
@@ -1407,19 +1433,19 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
         Kt[rr * KT8 + cc] = k64[i][j][v];
       }
 }
-template <int NW>
+template <int NW, bool BUF>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2))) void k_syrk_i8(
     const int8_t* __restrict__ Xd, long xplane, const int* __restrict__ Xe, long ldk, int nbf, int naux,
     int ntiles, int nsplit, double* __restrict__ Kpart) {
-  k_syrk_i8_body<NW>(grid_vb(), Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
+  k_syrk_i8_body<NW, BUF>(grid_vb(), Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
 }
-template <int NW>
+template <int NW, bool BUF>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2, NW / 2)))
 void k_syrk_i8_list(const ItemSyrk* __restrict__ items,
     const int* __restrict__ first, int n) {
   const int i = list_item(first, n, (int)blockIdx.x);
   const ItemSyrk& a = items[i];
-  k_syrk_i8_body<NW>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
+  k_syrk_i8_body<NW, BUF>(list_vb((int)blockIdx.x - first[i], a.gx, a.gy, a.gz),
       a.Xd, a.xplane, a.Xe, a.ldk, a.nbf, a.naux, a.ntiles, a.nsplit, a.Kpart);
 }
 
@@ -1728,11 +1754,14 @@ hipError_t jpart_slots_item(int nd, int nsplit, const int8_t* Bs, long splane, l
   return hipSuccess;
 }
 
+// the exchange's buffer-load form: 32-bit offsets cover a digit plane and the exponents
+bool syrk_buf(long xplane, int nbf, int naux) { return xplane <= 0x7fffffffL && 4L * naux * nbf <= 0x7fffffffL; }
+
 hipError_t syrk_i8_item(int ntiles, int nsplit, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
                         int naux, double* Kpart, ListRec* r) {
   if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
   r->kind = kListSyrk;
-  r->cfg = 4;
+  r->cfg = syrk_buf(xplane, nbf, naux) ? 5 : 4;
   r->blocks = ntiles * nsplit;  // a multiple of 8: the XCD numbering of k_syrk_i8 holds within the item
   r->threads = 256;
   put_args(r, ItemSyrk{Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart, ntiles * nsplit, 1, 1});
@@ -1784,7 +1813,10 @@ hipError_t launch_list(int kind, int cfg, const void* items, const int* first, i
       break;
     }
     case kListSyrk:
-      k_syrk_i8_list<4><<<blocks, 256, 0, s>>>((const ItemSyrk*)items, first, n);
+      if (cfg == 5)
+        k_syrk_i8_list<4, true><<<blocks, 256, 0, s>>>((const ItemSyrk*)items, first, n);
+      else
+        k_syrk_i8_list<4, false><<<blocks, 256, 0, s>>>((const ItemSyrk*)items, first, n);
       break;
     default: return hipErrorInvalidValue;
   }
@@ -1795,7 +1827,12 @@ hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* X
                           long ldk, int nbf, int naux, double* Kpart) {
   if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
   // four waves of 32 x 32 (two waves per SIMD, two blocks per CU)
-  k_syrk_i8<4><<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart);
+  if (syrk_buf(xplane, nbf, naux))
+    k_syrk_i8<4, true><<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit,
+                                                                    Kpart);
+  else
+    k_syrk_i8<4, false><<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit,
+                                                                     Kpart);
   return hipGetLastError();
 }
 
