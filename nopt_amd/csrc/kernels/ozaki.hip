@@ -977,21 +977,28 @@ __device__ __forceinline__ void k_gamma_slots_body(const VB vb, const int8_t* __
   static_assert(!SEG || SPL == 2, "a segment is two slots");
   int mrow[SPL], ncol[SPL];
   double dv[ND][SPL][8];
+  // every lane loads (empty slots and columns beyond the row read a valid
+  // element and weigh it zero), so that the loads issue back to back
+  bool okk[SPL];
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
     // SEG: the table lists 16-column segments, lane's slots are its two halves
     const int sl = SEG ? sb * 64 + lane : sb * 64 * SPL + k * 64 + lane;
-    const int code = sl < nslots ? slots[sl] : -1;
-    mrow[k] = code >= 0 ? code >> 16 : 0;  // row 0 stands in for an empty slot (zero weights)
-    ncol[k] = code >= 0 ? (code & 0xffff) + (SEG ? 8 * k : 0) : 0;
+    okk[k] = sl < nslots;
+    const int code = slots[min(sl, nslots - 1)];
+    mrow[k] = okk[k] ? code >> 16 : 0;  // row 0 stands in for an empty slot (zero weights)
+    ncol[k] = okk[k] ? (code & 0xffff) + (SEG ? 8 * k : 0) : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < SPL; ++k)
 #pragma unroll
     for (int d = 0; d < ND; ++d)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int n = ncol[k] + j, m = mrow[k];
-        dv[d][k][j] = (code >= 0 && n <= m) ? (n < m ? 2.0 : 1.0) * Dp[d * dstride + (long)m * ldd + n] : 0.0;
+        const double x = Dp[d * dstride + (long)m * ldd + min(n, m)];
+        dv[d][k][j] = (okk[k] && n <= m) ? (n < m ? 2.0 : 1.0) * x : 0.0;
       }
-  }
   Dig8 cur[SPL], nxt[SPL];
   int ecur[SPL], enxt[SPL];
   int boff[SPL];  // element offset of the slot in a packed B_Q
@@ -1117,7 +1124,7 @@ __device__ __forceinline__ void k_jpart_slots_body(const VB vb, const int8_t* __
 #pragma unroll
   for (int k = 0; k < SPL; ++k) {
     const int sl = SEG ? sb * 256 + (int)threadIdx.x : sb * 256 * SPL + k * 256 + threadIdx.x;
-    const int code = sl < nslots ? slots[sl] : -1;
+    const int code = sl < nslots ? slots[min(sl, nslots - 1)] : -1;
     live[k] = code >= 0;
     mrow[k] = live[k] ? code >> 16 : 0;  // row 0 stands in for an empty slot (result not written)
     ncol[k] = live[k] ? (code & 0xffff) + (SEG ? 8 * k : 0) : 0;
