@@ -1367,44 +1367,29 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
 #pragma unroll
         for (int p = 0; p < 4; ++p) av[i][p] = *reinterpret_cast<const uint4*>(As + (p * KT8 + r) * KS + 16 * swz(r, lc));
       }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      // the column operand one digit plane at a time, the next plane's LDS
+      // read issued before the current plane's products (two registers in
+      // turn, scheduling groups: one LDS read, then the products)
+      auto bplane = [&](int j, int l) {
         const int cidx = wc * 32 + j * 16 + lr;
-        const int8_t* cp = Bs + cidx * KS + 16 * swz(cidx, lc);
-        uint4 bv = *reinterpret_cast<const uint4*>(cp);
+        return *reinterpret_cast<const uint4*>(Bs + cidx * KS + 16 * swz(cidx, lc) + l * KT8 * KS);
+      };
+      uint4 bq[2];
+      bq[0] = bplane(0, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 4 * IT + 1, 0);
+      static_for<8>([&](auto S) {
+        constexpr int s_ = decltype(S)::value, j = s_ / 4, l = s_ % 4;
+        if constexpr (s_ + 1 < 8) bq[(s_ + 1) & 1] = bplane((s_ + 1) / 4, (s_ + 1) % 4);
+        const uint4 bv = bq[s_ & 1];
 #pragma unroll
         for (int i = 0; i < IT; ++i) {
           v4i_t* A = acc[i][j];
-          A[0] = mfma_i8(av[i][0], bv, A[0]);
-          A[1] = mfma_i8(av[i][1], bv, A[1]);
-          A[2] = mfma_i8(av[i][2], bv, A[2]);
-          A[3] = mfma_i8(av[i][3], bv, A[3]);
-        }
-        bv = *reinterpret_cast<const uint4*>(cp + KT8 * KS);
 #pragma unroll
-        for (int i = 0; i < IT; ++i) {
-          v4i_t* A = acc[i][j];
-          A[1] = mfma_i8(av[i][0], bv, A[1]);
-          A[2] = mfma_i8(av[i][1], bv, A[2]);
-          A[3] = mfma_i8(av[i][2], bv, A[3]);
-          A[4] = mfma_i8(av[i][3], bv, A[4]);
+          for (int k = 0; k + l <= 4 && k < 4; ++k) A[k + l] = mfma_i8(av[i][k], bv, A[k + l]);
         }
-        bv = *reinterpret_cast<const uint4*>(cp + 2 * KT8 * KS);
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-          v4i_t* A = acc[i][j];
-          A[2] = mfma_i8(av[i][0], bv, A[2]);
-          A[3] = mfma_i8(av[i][1], bv, A[3]);
-          A[4] = mfma_i8(av[i][2], bv, A[4]);
-        }
-        bv = *reinterpret_cast<const uint4*>(cp + 3 * KT8 * KS);
-#pragma unroll
-        for (int i = 0; i < IT; ++i) {
-          v4i_t* A = acc[i][j];
-          A[3] = mfma_i8(av[i][0], bv, A[3]);
-          A[4] = mfma_i8(av[i][1], bv, A[4]);
-        }
-      }
+        if constexpr (s_ + 1 < 8) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, IT * (l < 2 ? 4 : 5 - l), 0);
+      });
       if (qend) {
         // combine the class sums of this auxiliary function: with |digit| <= 64
         // and ldk <= 1024, hi = 128 A0 + A1 and mid = 128 A2 + A3 are exact in
