@@ -222,8 +222,8 @@ int nopt_jk_energy_gradient_device(nopt_jk_ctx* ctx, const double* dH, const dou
 /* Tile and split tuning. The build divides the auxiliary index into Q-splits
  * (partial sums over Q ranges, added in FP64 at the end): params[6] =
  * {FP64/FP32/bf16 exchange splits, FP64 J splits (0: the fewest the kernel's
- * gamma cache allows, at least 16), integer exchange splits (a multiple of
- * 8), integer gamma splits, integer J splits, and where the integer Coulomb
+ * gamma cache allows, at least 16), integer exchange splits, integer gamma
+ * splits, integer J splits, and where the integer Coulomb
  * pair starts on its side stream: 0 after the half-transform, 1 gamma beside
  * the half-transform and J after it, 2 both beside it}. create picks them from the
  * device's CU count; nopt_jk_autotune times `reps` device builds of (dD, dC)

@@ -1663,9 +1663,9 @@ int nopt_jk_create_ex(int device, long nbf, long naux, int flags, nopt_jk_ctx** 
   c->ntiles = c->ktn * (c->ktn + 1) / 2;
   c->nsk = std::max(1, std::min((int)naux, (1024 + c->ntiles - 1) / c->ntiles));
   {
-    // integer exchange: Q-splits in multiples of 8 (one contiguous range
-    // per XCD), sized so that the blocks of an XCD fill whole rounds of its
-    // resident slots (two blocks per CU)
+    // integer exchange: by default Q-splits in multiples of 8 (as many
+    // blocks on every XCD), sized so that the blocks of an XCD fill whole
+    // rounds of its resident slots (two blocks per CU)
     const int kt8 = (int)((nbf + nopt::kSyrkI8Tile - 1) / nopt::kSyrkI8Tile);
     c->ntiles8 = kt8 * (kt8 + 1) / 2;
     int ncu = 0;
@@ -1834,7 +1834,7 @@ int nopt_jk_set_tuning(nopt_jk_ctx* c, const int* params) {
   if (c->pins > 0)
     return set_error(NOPT_ESTATE, "nopt_jk_set_tuning: a captured HIP graph refers to this context's buffers");
   const int na = (int)c->naux;
-  if (params[0] < 1 || params[0] > na || params[1] < 0 || params[1] > na || params[2] < 8 || params[2] % 8 ||
+  if (params[0] < 1 || params[0] > na || params[1] < 0 || params[1] > na || params[2] < 1 ||
       params[2] > std::max(8, na) || params[3] < 1 || params[3] > na || params[4] < 1 || params[4] > na ||
       params[5] < 0 || params[5] > 2)
     return set_error(NOPT_EINVAL, "nopt_jk_set_tuning: split counts out of range (see nopt_jk.h)");
@@ -1891,6 +1891,13 @@ int nopt_jk_autotune(nopt_jk_ctx* c, const double* dD, const double* dC, long no
   if (i8) {
     knobs[nk] = &c->nsk8;
     for (int x = 8; x <= std::min(128, std::max(8, na)); x += 8) cand[nk].push_back(x);
+    // and the largest counts that give each CU at most 1, 2 or 3 exchange
+    // blocks: with multiples of 8 alone some CUs can hold one block more
+    // than the rest, and the kernel waits for them
+    for (int b = 1; b <= 3; ++b) {
+      const int x = b * c->ncu / std::max(1, c->ntiles8);
+      if (x >= 1 && x <= std::max(8, na) && x % 8) cand[nk].push_back(x);
+    }
     ++nk;
     knobs[nk] = &c->nsj8;
     cand[nk++] = pow2(1, std::min(64, na));

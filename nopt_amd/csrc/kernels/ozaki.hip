@@ -1225,9 +1225,10 @@ __device__ __forceinline__ void tri_tile(int t, int& bi, int& bj) {
 constexpr int KT8 = 64;
 //
 // Blocks are numbered so that each XCD (workgroups go to the 8 XCDs round
-// robin) owns a contiguous range of Q-splits with all their tiles: the tiles
-// of one split read the same X rows at about the same time, from that XCD's
-// L2. The grid is 1-D, ntiles * nsplit blocks, nsplit a multiple of 8.
+// robin) owns a contiguous range of the (tile, split) pairs, tile fastest:
+// the tiles of one split read the same X rows at about the same time, from
+// that XCD's L2. The grid is 1-D, ntiles * nsplit blocks; when that is not a
+// multiple of 8, the first (ntiles * nsplit) % 8 XCDs take one pair more.
 // BUF: the operand and exponent loads as buffer loads (32-bit offsets: a
 // digit plane and the exponents below 2^31 bytes, which the launchers check),
 // else as loads through 64-bit addresses.
@@ -1242,8 +1243,8 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
   __shared__ __attribute__((aligned(16))) int8_t sm[2][2 * TB];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6, wr = w >> 1, wc = w & 1;
   const int lr = lane & 15, lc = lane >> 4;
-  const int per_xcd = ntiles * nsplit / 8;
-  const int gidx = (vb.x % 8) * per_xcd + vb.x / 8;
+  const int total = ntiles * nsplit, per_xcd = total / 8, xcd = vb.x % 8;
+  const int gidx = xcd * per_xcd + min(xcd, total % 8) + vb.x / 8;
   const int tile = gidx % ntiles, sp = gidx / ntiles, ns = nsplit;
   int bi, bj;
   tri_tile(tile, bi, bj);
@@ -1751,10 +1752,10 @@ bool syrk_buf(long xplane, int nbf, int naux) { return xplane <= 0x7fffffffL && 
 
 hipError_t syrk_i8_item(int ntiles, int nsplit, const int8_t* Xd, long xplane, const int* Xe, long ldk, int nbf,
                         int naux, double* Kpart, ListRec* r) {
-  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
+  if (nsplit < 1 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
   r->kind = kListSyrk;
   r->cfg = syrk_buf(xplane, nbf, naux) ? 5 : 4;
-  r->blocks = ntiles * nsplit;  // a multiple of 8: the XCD numbering of k_syrk_i8 holds within the item
+  r->blocks = ntiles * nsplit;
   r->threads = 256;
   put_args(r, ItemSyrk{Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit, Kpart, ntiles * nsplit, 1, 1});
   return hipSuccess;
@@ -1817,7 +1818,7 @@ hipError_t launch_list(int kind, int cfg, const void* items, const int* first, i
 
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart) {
-  if (nsplit % 8 != 0 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
+  if (nsplit < 1 || ntiles <= 0 || ldk % 64 != 0 || ldk > 1024) return hipErrorInvalidValue;
   // four waves of 32 x 32 (two waves per SIMD, two blocks per CU)
   if (syrk_buf(xplane, nbf, naux))
     k_syrk_i8<4, true><<<(unsigned)(ntiles * nsplit), 256, 0, s>>>(Xd, xplane, Xe, ldk, nbf, naux, ntiles, nsplit,

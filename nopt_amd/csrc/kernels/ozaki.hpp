@@ -126,8 +126,8 @@ hipError_t launch_jpart_slots(int nd, int nsplit, hipStream_t s, const int8_t* B
                               const double* gamma, long gstride, double* Jpart, long jstride);
 
 // Exchange partials over 64x64 lower-triangle tiles: Kpart[split][tile] for
-// ntiles tiles and nsplit Q-splits (a multiple of 8, one contiguous range of
-// splits per XCD).
+// ntiles tiles and nsplit Q-splits (each XCD a contiguous range of the
+// (tile, split) pairs).
 constexpr int kSyrkI8Tile = 64;
 hipError_t launch_syrk_i8(int ntiles, int nsplit, hipStream_t s, const int8_t* Xd, long xplane, const int* Xe,
                           long ldk, int nbf, int naux, double* Kpart);

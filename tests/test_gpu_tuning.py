@@ -43,7 +43,7 @@ def test_autotune_keeps_results(prec):
         ctx.set_tuning(**t0)
         assert ctx.tuning() == t0
         with pytest.raises(L.NoptError):
-            ctx.set_tuning(nsk8=12)
+            ctx.set_tuning(nsk8=0)
         with pytest.raises(L.NoptError):
             ctx.set_tuning(nsk=0)
         dD.free()
