@@ -627,7 +627,8 @@ def main():
     runs, tuning = {}, {}
     for p in modes:
         # split counts chosen by the autotuner on this density, then the timed builds
-        tuning[p], _ = ctx.autotune(dD.ptr, dC.ptr, nocc, prec=p, reps=8)
+        # (16 builds per candidate: the counts that matter differ by a few per cent)
+        tuning[p], _ = ctx.autotune(dD.ptr, dC.ptr, nocc, prec=p, reps=16)
         runs[p] = time_builds(ctx, dD, dC, nocc, dJ, dK, p, args.steps, args.warmup, synchronize, timing=True)
         log(f"[bench] {p}: {runs[p][0]:.1f} builds/s ({runs[p][1]:.3f} ms/build); phases {runs[p][2]}")
     if args.prec == "auto":
