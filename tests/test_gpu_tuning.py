@@ -48,3 +48,31 @@ def test_autotune_keeps_results(prec):
             ctx.set_tuning(nsk=0)
         dD.free()
         dC.free()
+
+
+def test_exchange_split_counts_not_multiples_of_8():
+    """The integer exchange takes any Q-split count (the XCD ranges of its
+    blocks then differ in size by one): K for 1, 3, 7 and 13 splits equals
+    the 8-split K to the rounding of the FP64 sums over the splits, and a
+    build repeated at one count is the same to the bit."""
+    sym, xyz = G.water_cluster(6, 3)
+    m = Molecule.from_angstrom(sym, xyz, basis="cc-pvdz", aux="cc-pvdz-jkfit")
+    rng = np.random.default_rng(7)
+    C, _ = np.linalg.qr(rng.standard_normal((m.nbf, m.nbf)))
+    Co = np.ascontiguousarray(C[:, :m.nocc].T)
+    D = Co.T @ Co
+    with JKContext(m.nbf, m.naux) as ctx:
+        ctx.set_B(m.eri3_packed(), m.metric())
+        t = ctx.tuning()
+        ctx.set_tuning(**dict(t, nsk8=8))
+        J8, K8 = ctx.build(D, Co, m.nocc, prec="int8")
+        for ns in (1, 3, 7, 13):
+            ctx.set_tuning(**dict(t, nsk8=ns))
+            assert ctx.tuning()["nsk8"] == ns
+            J, K = ctx.build(D, Co, m.nocc, prec="int8")
+            J2, K2 = ctx.build(D, Co, m.nocc, prec="int8")
+            dk = float(np.abs(K - K8).max() / np.abs(K8).max())
+            say("nsk8", ns, "rel max|dK| vs 8 splits", dk)
+            assert np.array_equal(K, K2) and np.array_equal(J, J2)
+            assert np.array_equal(J, J8)
+            assert dk < 1e-13
