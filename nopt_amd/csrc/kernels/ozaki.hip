@@ -376,10 +376,14 @@ __device__ __forceinline__ int row_max16(int x) {
 
 // ------------------------------------------------------------ half-transform
 // LDS position of 16-byte chunk c (of a 64-byte k-slab) in row r: an XOR
-// swizzle under which the 16 rows x 4 chunks of one MFMA operand read, and
-// the contiguous rows written by the loaders, fall on distinct bank groups for
-// both 32 and 64 four-byte banks.
-__device__ __forceinline__ int swz(int r, int c) { return c ^ (((r >> 1) + (r >> 3)) & 3); }
+// swizzle under which the operand reads (ds_read_b128, lane l: row l % 16 of
+// a 16-row block, chunk l / 16) hit no bank conflict on gfx950. Measured with
+// SQ_LDS_BANK_CONFLICT: c ^ ((r >> 1) & 3), and 16 rows read 4 lanes per
+// row in order, are free of conflicts; c ^ (r & 3), c ^ ((r >> 2) & 3),
+// c ^ (((r >> 1) + (r >> 3)) & 3) and no swizzle at all cost 4 cycles per
+// read. The loaders write whole 64-byte rows per 4 (or 16) lanes, which no
+// swizzle changes.
+__device__ __forceinline__ int swz(int r, int c) { return c ^ ((r >> 1) & 3); }
 
 // X[Q][m][col] = 2^(e_Qm + f_col - 12) sum_{k+l<=4} 2^(-7(k+l)) <Bd_k[Q][m][:], Cd_l[col][:]>
 // for k, l < 4 (13 digit pairs; the products of equal k+l share one exact
