@@ -1258,10 +1258,6 @@ __device__ __forceinline__ void k_syrk_i8_body(const VB vb,
   // loader: row (t/4)%64 of operand tile (t/256 + o) (o < OPT; tile 0 rows
   // bi, tile 1 rows bj), chunk t%4, all 4 planes
   const int ar = (t >> 2) & 63, ac = t & 3, op0 = (OPT == 2) ? 0 : (t >> 8);
-  // loads from a base address the same for the whole block (the slab's,
-  // in scalar registers) plus the lane's row and chunk as a 32-bit offset;
-  // rows beyond nbf load row nbf - 1 and store zeros, so every lane issues
-  // the same loads and the waits count them exactly
   // rows beyond nbf load row nbf - 1 and store zeros: every lane issues the
   // same loads, so that the waits count them exactly. Buffer form: a
   // resource per digit plane, the lane's row and chunk as the vector offset,
