@@ -535,6 +535,9 @@ def main():
                     help="headline mode; auto = fastest mode with |dE| < 1 uEh against FP64")
     ap.add_argument("--alkane", type=int, default=20, help="carbon count of the all-trans alkane")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tuning", default="", metavar="KEY=N,...",
+                    help="split counts for the modes other than fp64 instead of the autotuner, e.g. "
+                         "nsk8=8,nsg=64,nsj8=8,cplace=0 (keys of JKContext.TUNING_KEYS)")
     ap.add_argument("--modes", default="fp64,int8",
                     help="comma-separated modes to time (fp64 is always built as the reference); the FP32 and "
                          "bf16 modes (fp32, bf16x6, bf16x3) are slower and less accurate than int8 at C20H42 and "
@@ -610,6 +613,12 @@ def main():
 
     # accuracy of each mode against the FP64 build on the same density
     modes = tuple(["fp64"] + [p for p in args.modes.split(",") if p and p != "fp64"])
+    fixed_tuning = {}
+    for kv in filter(None, args.tuning.split(",")):
+        k, v = kv.split("=")
+        if k not in JKContext.TUNING_KEYS:
+            raise SystemExit(f"--tuning: unknown key {k} (one of {', '.join(JKContext.TUNING_KEYS)})")
+        fixed_tuning[k] = int(v)
     acc = {}
     ref = None
     for p in modes:
@@ -628,7 +637,11 @@ def main():
     for p in modes:
         # split counts chosen by the autotuner on this density, then the timed builds
         # (16 builds per candidate: the counts that matter differ by a few per cent)
-        tuning[p], _ = ctx.autotune(dD.ptr, dC.ptr, nocc, prec=p, reps=16)
+        if fixed_tuning and p != "fp64":
+            ctx.set_tuning(**fixed_tuning)
+            tuning[p] = ctx.tuning()
+        else:
+            tuning[p], _ = ctx.autotune(dD.ptr, dC.ptr, nocc, prec=p, reps=16)
         runs[p] = time_builds(ctx, dD, dC, nocc, dJ, dK, p, args.steps, args.warmup, synchronize, timing=True)
         log(f"[bench] {p}: {runs[p][0]:.1f} builds/s ({runs[p][1]:.3f} ms/build); phases {runs[p][2]}")
     if args.prec == "auto":
